@@ -1,0 +1,155 @@
+"""Throughput of the batched layer-stack solve on one MI355X.
+
+Workload: the 2D rectangular-pillar batch (Nx = Ny = 7: 15 x 15 = 225 orders,
+450 x 450 layer matrices; 4 patterned layers between air and eps = 2.25;
+256 frequencies x 8 incidence directions = 2048 solves, TE and TM each).
+One step = one execute of the whole 2048-instance batch with inputs resident
+in device memory.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample S] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "layer-stack solves/sec at 2D 15x15 orders; max |dR,dT| vs FP64 oracle"
+FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and MFMA)
+
+
+def eig_flops(n):
+    # canonical complex zgeev work (Hessenberg + Schur + vectors): 100 n^3 real flops
+    return 100.0 * n ** 3
+
+
+def solve_flops(h, n_pat, n_layers):
+    # canonical FP64 flop count of one layer-stack solve (both polarisations)
+    n = 2 * h
+    return n_pat * ((8.0 / 3.0 + 16.0) * h ** 3 + 108.0 * n ** 3) + \
+        (n_layers - 1) * (2 * (8.0 / 3.0 + 8.0) + 40.0) * n ** 3
+
+
+def _cpu_worker(args):
+    w, i = args
+    from threadpoolctl import threadpool_limits
+    from oracle import rcwa_oracle as o
+    with threadpool_limits(1):
+        th, ph = w["angles"][i]
+        t0 = time.perf_counter()
+        REF, TRN, _ = o.solve_stack(w["nX"], w["nY"], w["period"], w["period"], w["coord_x"],
+                                    w["coord_y"], list(w["eps"]), w["thickness"], w["lam"][i],
+                                    th, ph)
+        return i, REF, TRN, time.perf_counter() - t0
+
+
+def cpu_baseline(w, sample_idx, cores):
+    """FP64 oracle (numpy/LAPACK zgeev, the reference's operation sequence) on
+    a bounded sample of the same workload, one BLAS thread per worker process."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        out = pool.map(_cpu_worker, [(w, int(i)) for i in sample_idx])
+    wall = time.perf_counter() - t0
+    return out, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="instances timed on the CPU")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n-freq", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the batch shards "
+              "across GPUs by instance, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from openrcwa_amd import engine, workloads
+
+    w = workloads.pillar2d(n_freq=args.n_freq)
+    B = w["lam"].size
+    h = (2 * w["nX"] + 1) * (2 * w["nY"] + 1)
+    n = 2 * h
+    n_pat = len(w["eps"]) - 2
+    print(f"[bench] workload pillar2d: {B} solves, h={h}, n={n}, {n_pat} patterned layers",
+          file=sys.stderr, flush=True)
+    ctx = engine.Context(0)
+    plan = engine.Plan(ctx, w["nX"], w["nY"], w["coord_x"], w["coord_y"], w["thickness"],
+                       w["lam"], w["kx0"], w["ky0"], w["eps"], w["pols"])
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        plan.execute()
+        plan.sync()
+        print(f"[bench] warmup {i}: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    plan.sync()
+    t0 = time.perf_counter()
+    eig_ms = 0.0
+    for i in range(args.steps):
+        plan.execute()
+        plan.sync()
+        eig_ms += plan.phase_ms(5)
+        print(f"[bench] step {i}: {plan.phase_ms(0) / 1e3:.2f} s (eig {plan.phase_ms(5) / 1e3:.2f} s)",
+              file=sys.stderr, flush=True)
+    plan.sync()
+    dt = time.perf_counter() - t0
+    res = plan.results()
+    value = args.steps * B / dt
+    phases = {k: plan.phase_ms(i) for i, k in enumerate(
+        ["total", "convolution", "modes", "cascade", "plane_wave", "eigensolver"])}
+
+    # roofline of the dominant kernel family: the batched eigensolver
+    eig_avg_s = eig_ms / args.steps / 1e3
+    eig_alg = eig_flops(n) * B * n_pat
+    achieved = eig_alg / eig_avg_s / 1e12
+    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "batched zgeev (Hessenberg + Schur QR + eigenvectors), 100 n^3 flop/matrix",
+            "launch_ms": eig_ms / args.steps}
+
+    out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 (complex128)", "data": "synthetic (seeded splitmix64)",
+           "config": {"workload": "pillar2d: Nx=Ny=7 (225 orders, n=450), 4 patterned layers, "
+                                  "256 freqs x 8 angles = 2048 solves, TE+TM",
+                      "batch": B, "orders": h, "matrix_n": n},
+           "phase_ms": phases, "status_nonzero": int((res.status != 0).sum()),
+           "alg_tflops_whole_solve": solve_flops(h, n_pat, n_pat + 2) * B / (dt / args.steps) / 1e12,
+           "roofline": roof}
+
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        S = args.cpu_sample or 2 * cores
+        rng = np.random.default_rng(0)
+        sample = np.sort(rng.choice(B, size=min(S, B), replace=False))
+        print(f"[bench] cpu baseline: {len(sample)} solves on {cores} processes", file=sys.stderr,
+              flush=True)
+        outs, wall = cpu_baseline(w, sample, cores)
+        err = 0.0
+        for i, REF, TRN, _ in outs:
+            for p in range(2):
+                err = max(err, np.abs(res.ref[i, p] - REF[p]).max(),
+                          np.abs(res.trn[i, p] - TRN[p]).max())
+        per = float(np.median([o_[3] for o_ in outs]))
+        out["max_abs_dRdT_vs_fp64_oracle"] = err
+        out["cpu_baseline"] = {"value": len(sample) / wall, "unit": "solves/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"{len(sample)} random instances of the same 2048-solve "
+                                         f"batch; numpy/LAPACK zgeev oracle, 1 BLAS thread per "
+                                         f"process; median {per:.2f} s per solve per core"}
+        out["gpu_over_cpu"] = value / (len(sample) / wall)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/*
+ * openrcwa_gpu.h -- C ABI of the MI355X RCWA layer-stack engine.
+ *
+ * Two entry families:
+ *
+ *  1. rcwa_run(FILE*): the reference's C entry point for the .ofd RCWA path,
+ *     same globals, same rcwa_efficiency.csv / rcwa_orders.csv bytes and log
+ *     lines.  Replaces sol/rcwa_bridge.cpp:145 (declared in the reference at
+ *     include/orcwa_rcwa.h:72-74, called from src/sol_Main.c:81-82).
+ *
+ *  2. A batched ABI behind the RCWASolver facade: one call solves many
+ *     independent (wavelength, Bloch vector, permittivity) instances of one
+ *     stack layout.  It replaces the per-wavelength sequence
+ *       RCWASolver(nX,nY) / addLayer / disablePML / setBlochWavevector /
+ *       solve / generateHorizontalPlaneWave / scatterPlaneWave
+ *     (rcwa/RCWASolver.h:67-185, rcwa/RCWASolver.cpp:108-148, 1778-1908) that
+ *     sol/rcwa_bridge.cpp:244-334 and rcwa/RCWADriver.cpp:282-368 run one
+ *     wavelength at a time.
+ *
+ * Plain pointers and sizes only; no exceptions cross the ABI; every function
+ * returns an orcwa_error code.  A context owns one HIP stream and its device
+ * memory; calls on one context must not overlap, distinct contexts are
+ * independent.
+ */
+#ifndef OPENRCWA_GPU_H
+#define OPENRCWA_GPU_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+enum orcwa_error {
+  ORCWA_OK = 0,
+  ORCWA_ERR_ARG = 1,          /* invalid argument (message on stderr) */
+  ORCWA_ERR_HIP = 2,          /* HIP runtime error, or no usable GPU */
+  ORCWA_ERR_NOMEM = 3,        /* device allocation failed */
+  ORCWA_ERR_UNSUPPORTED = 4   /* input outside what the engine implements */
+};
+
+/* per-instance status bits (orcwa_result.status) */
+enum orcwa_status {
+  ORCWA_STATUS_OK = 0,
+  ORCWA_STATUS_EIG_RESIDUAL = 1,  /* a layer's ||AW-W diag(l)||/||A|| exceeded 1e-8
+                                     (acceptance test of rcwa/MKLEigenSolver.hpp:66-79) */
+  ORCWA_STATUS_GRAZING = 2,       /* kz == 0 for some order, or kz_inc < 1e-12 k0: the
+                                     reference returns empty REF/TRN
+                                     (rcwa/RCWASolver.cpp:1843-1862, 1898-1901);
+                                     ref/trn are left zero */
+  ORCWA_STATUS_SINGULAR = 4,      /* exactly singular pivot in an interface/cascade solve */
+  ORCWA_STATUS_NOCONV = 8         /* the QR iteration hit its iteration limit */
+};
+
+typedef struct orcwa_gpu_context orcwa_gpu_context;
+typedef struct orcwa_plan orcwa_plan;
+
+/* Stack layout shared by every instance of a batch.  Mirrors what
+ * RCWASolver(nX, nY) + addLayer(Layer(coordX, coordY, eps)) fixes
+ * (rcwa/RCWASolver.h:67-78, rcwa/Layer.h:29-37): all layers share the cell
+ * edges, as RCWASolver::addLayer requires (rcwa/RCWASolver.cpp:125-133). */
+typedef struct {
+  int nx_harm;               /* Nx: x orders -Nx..Nx */
+  int ny_harm;               /* Ny: y orders -Ny..Ny (0 = 1D grating) */
+  int n_cx, n_cy;            /* number of cell edges along x, y (>= 2) */
+  const double *coord_x;     /* [n_cx] sorted edges; Lx = last - first */
+  const double *coord_y;     /* [n_cy] sorted edges; Ly = last - first */
+  int n_layers;              /* L >= 2; layer 0 = incidence half-space, L-1 = exit */
+  const double *thickness;   /* [L]; entries 0 and L-1 are not used */
+} orcwa_stack;
+
+/* One solve instance (one iteration of sol/rcwa_bridge.cpp:244-334). */
+typedef struct {
+  double lambda;             /* wavelength, in the length unit of the coordinates */
+  double kx0, ky0;           /* Bloch offsets (RCWASolver::setBlochWavevector) */
+  const double *eps;         /* [L][(n_cy-1)*(n_cx-1)] complex (re,im); cell (iy,ix)
+                                at iy*(n_cx-1)+ix, row = y cell (rcwa/Layer.cpp:29-33).
+                                Instances passing the same pointer share convolution
+                                matrices. */
+  int n_pol;                 /* 1..8 incident polarisations */
+  const double *pol;         /* [n_pol][4] = Re px, Im px, Re py, Im py: the transverse
+                                amplitudes of RCWASolver::generateHorizontalPlaneWave */
+} orcwa_instance;
+
+/* Output of one instance.  Orders k = (2Ny+1)(m+Nx) + (n+Ny), h = (2Nx+1)(2Ny+1)
+ * (rcwa/RCWASolver.cpp:255-264). */
+typedef struct {
+  double *ref;               /* [n_pol][h] REF of scatterPlaneWave, or NULL */
+  double *trn;               /* [n_pol][h] TRN, or NULL */
+  double *amp;               /* optional [2][4][h] complex (re,im): for incident basis
+                                s_x=(1,0), s_y=(0,1) the reflected (rx, ry) and
+                                transmitted (tx, ty) harmonic amplitudes; or NULL */
+  int status;                /* orcwa_status bits */
+} orcwa_result;
+
+/* Context: device selection, stream, device memory pool. */
+int orcwa_gpu_create(orcwa_gpu_context **ctx, int device);
+int orcwa_gpu_destroy(orcwa_gpu_context *ctx);
+
+/* One-shot batched solve: plan + execute + results + destroy.
+ * Replaces the per-wavelength loop of sol/rcwa_bridge.cpp:244-334 and
+ * rcwa/RCWADriver.cpp:282-368 (RCWASolver::solve + scatterPlaneWave). */
+int orcwa_gpu_solve_batch(orcwa_gpu_context *ctx, const orcwa_stack *stack,
+                          const orcwa_instance *inst, int count, orcwa_result *out);
+
+/* Split form for callers that reuse one batch layout (benchmarks, sweeps):
+ * plan_create copies all inputs to device memory and allocates; execute runs
+ * the whole batch on the context stream (asynchronous); sync waits;
+ * results copies the outputs back.  The plan keeps pointers to nothing the
+ * caller owns. */
+int orcwa_plan_create(orcwa_gpu_context *ctx, const orcwa_stack *stack,
+                      const orcwa_instance *inst, int count, orcwa_plan **plan);
+int orcwa_plan_execute(orcwa_plan *plan);
+int orcwa_plan_sync(orcwa_plan *plan);
+int orcwa_plan_results(orcwa_plan *plan, orcwa_result *out);
+int orcwa_plan_destroy(orcwa_plan *plan);
+
+/* Elapsed GPU time (ms) of named phases of the last execute, measured with HIP
+ * events on the plan's stream.  phase: 0 total, 1 convolution matrices,
+ * 2 P/Q + eigensolve + V, 3 interfaces + cascade, 4 plane-wave R/T,
+ * 5 eigensolver only.  Returns a negative value if unknown. */
+double orcwa_plan_phase_ms(orcwa_plan *plan, int phase);
+
+/* Layout-compatible records of the .ofd RCWA input globals
+ * (include/orcwa_rcwa.h:27-35 and :45-52 of the reference): same field order
+ * and types, so rcwa_run reads the arrays the reference's input module fills. */
+typedef struct {
+  double eps1, eps2;         /* real parts inside / outside the eps1 rectangle */
+  double eps1i, eps2i;       /* imaginary parts (loss > 0, exp(-i w t)) */
+  double fill, fill_y;       /* eps1 rectangle [0, fill Lx] x [0, fill_y Ly] */
+  double thickness;          /* [m]; ignored for the two half-spaces */
+} orcwa_rcwalayer;
+
+typedef struct {
+  int layer;                 /* 0-based layer */
+  int side;                  /* 1 = eps1, 2 = eps2 */
+  double einf, ae, be, ce;   /* eps = einf + sum ae^2 / (ce^2 - w^2 - i be w) */
+} orcwa_rcwapole;
+
+/* The reference's .ofd entry point (sol/rcwa_bridge.cpp:145; declared at
+ * include/orcwa_rcwa.h:72-74).  Reads the RCWA globals of
+ * include/orcwa_rcwa.h:54-70 (defined by the caller's input module,
+ * sol/input_data.c:13-24) and NFreq1/Freq1/Title (sol/rcwa_bridge.cpp:35-40),
+ * writes rcwa_efficiency.csv and rcwa_orders.csv in the working directory and
+ * log lines through monitor1().  Returns 0 on success, 1 on error. */
+int rcwa_run(FILE *fp_log);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENRCWA_GPU_H */

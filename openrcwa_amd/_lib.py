@@ -1,0 +1,80 @@
+"""ctypes binding of ``include/openrcwa_gpu.h`` (the engine's C ABI).
+
+There is no CPU fallback: if ``libopenrcwa_gpu.so`` is missing or no GPU is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_void_p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libopenrcwa_gpu.so")
+
+ORCWA_OK = 0
+ERRORS = {1: "invalid argument", 2: "HIP runtime error / no GPU", 3: "out of device memory",
+          4: "unsupported input"}
+
+STATUS_EIG_RESIDUAL = 1
+STATUS_GRAZING = 2
+STATUS_SINGULAR = 4
+STATUS_NOCONV = 8
+
+
+class OrcwaStack(ctypes.Structure):
+    _fields_ = [("nx_harm", c_int), ("ny_harm", c_int), ("n_cx", c_int), ("n_cy", c_int),
+                ("coord_x", POINTER(c_double)), ("coord_y", POINTER(c_double)),
+                ("n_layers", c_int), ("thickness", POINTER(c_double))]
+
+
+class OrcwaInstance(ctypes.Structure):
+    _fields_ = [("lam", c_double), ("kx0", c_double), ("ky0", c_double),
+                ("eps", POINTER(c_double)), ("n_pol", c_int), ("pol", POINTER(c_double))]
+
+
+class OrcwaResult(ctypes.Structure):
+    _fields_ = [("ref", POINTER(c_double)), ("trn", POINTER(c_double)),
+                ("amp", POINTER(c_double)), ("status", c_int)]
+
+
+# every symbol include/openrcwa_gpu.h declares
+EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_solve_batch",
+            "orcwa_plan_create", "orcwa_plan_execute", "orcwa_plan_sync",
+            "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms", "rcwa_run"]
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load the native library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise EngineError(f"{p} is missing: build it with `python -m openrcwa_amd.build`")
+    lib = ctypes.CDLL(p)
+    lib.orcwa_gpu_create.argtypes = [POINTER(c_void_p), c_int]
+    lib.orcwa_gpu_destroy.argtypes = [c_void_p]
+    lib.orcwa_gpu_solve_batch.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
+                                          c_int, POINTER(OrcwaResult)]
+    lib.orcwa_plan_create.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
+                                      c_int, POINTER(c_void_p)]
+    lib.orcwa_plan_execute.argtypes = [c_void_p]
+    lib.orcwa_plan_sync.argtypes = [c_void_p]
+    lib.orcwa_plan_results.argtypes = [c_void_p, POINTER(OrcwaResult)]
+    lib.orcwa_plan_destroy.argtypes = [c_void_p]
+    lib.orcwa_plan_phase_ms.argtypes = [c_void_p, c_int]
+    lib.orcwa_plan_phase_ms.restype = c_double
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    if rc != ORCWA_OK:
+        raise EngineError(f"{what} failed: {ERRORS.get(rc, rc)}")

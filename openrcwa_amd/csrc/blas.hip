@@ -1,0 +1,379 @@
+// Batched dense complex FP64 linear algebra for gfx950: GEMM on FP64 MFMA
+// (v_mfma_f64_16x16x4f64), blocked LU with partial pivoting and the two
+// triangular solves.  These carry the mode-matching solves and the cascade
+// products of the layer-stack solve (rcwa/RCWASolver.cpp:378-614).
+#include "engine.h"
+
+namespace orcwa {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// GEMM: 64x64 output tile per workgroup (4 waves, 32x32 each = 2x2 MFMA
+// 16x16 sub-tiles), K staged through LDS 16 at a time, re/im split into
+// separate LDS planes so each MFMA reads one double per lane.
+// MFMA f64 16x16x4 lane layout: A[i][k] in lane i+16k, B[k][j] in lane
+// j+16k, D[i][j] in lane j+16(i/4), element i%4.
+// ---------------------------------------------------------------------------
+template <bool BETA0>
+__global__ __launch_bounds__(256) void k_zgemm(
+    int M, int N, int K, cd alpha, cd beta,
+    const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
+    const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
+    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
+  __shared__ double sAr[16][64];
+  __shared__ double sAi[16][64];
+  __shared__ double sBr[16][64];
+  __shared__ double sBi[16][64];
+
+  const int b = blockIdx.z;
+  const cd* __restrict__ A = Ap[b] + offA;
+  const cd* __restrict__ B = Bp[b] + offB;
+  cd* __restrict__ C = Cp[b] + offC;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
+
+  v4d accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
+
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j;
+      const int mm = e & 63, kk = e >> 6;
+      const int gm = m0 + mm, gk = k0 + kk;
+      cd v = mk(0.0, 0.0);
+      if (gm < M && gk < K) v = A[gm * rsA + gk * csA];
+      sAr[kk][mm] = v.re;
+      sAi[kk][mm] = sa * v.im;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j;
+      const int kk = e & 15, nn = e >> 4;
+      const int gn = n0 + nn, gk = k0 + kk;
+      cd v = mk(0.0, 0.0);
+      if (gn < N && gk < K) v = B[gk * rsB + gn * csB];
+      sBr[kk][nn] = v.re;
+      sBi[kk][nn] = sb * v.im;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < 16; kq += 4) {
+      const int ka = kq + (lane >> 4);
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = sAr[ka][wm + 16 * i + (lane & 15)];
+        ai[i] = sAi[ka][wm + 16 * i + (lane & 15)];
+        br[i] = sBr[ka][wn + 16 * i + (lane & 15)];
+        bi[i] = sBi[ka][wn + 16 * i + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int gn = n0 + wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N) {
+          cd v = alpha * mk(accR[i][j][r], accI[i][j][r]);
+          cd* p = C + gm * rsC + gn * csC;
+          if (!BETA0) v += beta * (*p);
+          *p = v;
+        }
+      }
+}
+
+void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
+                   const cd* const* A, long long offA, long long rsA, long long csA, int conjA,
+                   const cd* const* B, long long offB, long long rsB, long long csB, int conjB,
+                   cd beta, cd* const* C, long long offC, long long rsC, long long csC,
+                   int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  dim3 grid((M + 63) / 64, (N + 63) / 64, batch);
+  if (beta.re == 0.0 && beta.im == 0.0)
+    hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA, rsA,
+                       csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  else
+    hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA,
+                       rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting, right-looking, panel width NB.
+// ---------------------------------------------------------------------------
+static constexpr int NB = 32;
+
+// Panel factorisation of columns [k0, k0+nb) over rows [k0, n), one
+// workgroup per matrix, in global memory.  Row swaps are applied inside the
+// panel only.
+__global__ __launch_bounds__(256) void k_getf2_panel(int n, int k0, int nb, cd* const* Ap,
+                                                     int* ipiv, int* info, const int* info_map) {
+  __shared__ double s_val[256];
+  __shared__ int s_idx[256];
+  __shared__ cd s_inv;
+  __shared__ int s_piv;
+  const int b = blockIdx.x;
+  cd* __restrict__ A = Ap[b];
+  const int tid = threadIdx.x;
+  for (int j = 0; j < nb; ++j) {
+    const int c = k0 + j;
+    // pivot search: max cabs1 over rows c..n-1 of column c (first index wins ties)
+    double best = -1.0;
+    int bi = c;
+    for (int i = c + tid; i < n; i += 256) {
+      double v = cabs1(A[i + (long long)c * n]);
+      if (v > best) { best = v; bi = i; }
+    }
+    s_val[tid] = best;
+    s_idx[tid] = bi;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (tid < st) {
+        double o = s_val[tid + st];
+        int oi = s_idx[tid + st];
+        if (o > s_val[tid] || (o == s_val[tid] && oi < s_idx[tid])) {
+          s_val[tid] = o;
+          s_idx[tid] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      int p = s_idx[0];
+      s_piv = p;
+      ipiv[(long long)b * n + c] = p;
+      cd pv = A[p + (long long)c * n];
+      if (pv.re == 0.0 && pv.im == 0.0) {
+        atomicOr(&info[info_map ? info_map[b] : b], ORCWA_STATUS_SINGULAR);
+        s_inv = mk(0.0, 0.0);
+      } else {
+        s_inv = cinv(pv);
+      }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != c) {
+      for (int jj = tid; jj < nb; jj += 256) {
+        const long long col = (long long)(k0 + jj) * n;
+        cd t = A[c + col];
+        A[c + col] = A[p + col];
+        A[p + col] = t;
+      }
+    }
+    __syncthreads();
+    const cd inv = s_inv;
+    for (int i = c + 1 + tid; i < n; i += 256) A[i + (long long)c * n] = A[i + (long long)c * n] * inv;
+    __syncthreads();
+    // rank-1 update of the rest of the panel
+    const int ncol = nb - j - 1;
+    const int nrow = n - c - 1;
+    for (int e = tid; e < ncol * nrow; e += 256) {
+      const int jj = c + 1 + e / nrow;
+      const int i = c + 1 + e % nrow;
+      A[i + (long long)jj * n] -= A[i + (long long)c * n] * A[c + (long long)jj * n];
+    }
+    __syncthreads();
+  }
+}
+
+// Apply row swaps ipiv[k0..k0+nb) (in order) to columns [c0, c1) of an
+// n-row matrix with leading dimension n (one thread per column).
+__global__ void k_laswp(int n, int ncols_total, int k0, int nb, int c0, int c1, cd* const* Ap,
+                        const int* ipiv) {
+  const int b = blockIdx.y;
+  const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= c1 || col >= ncols_total) return;
+  cd* __restrict__ A = Ap[b] + (long long)col * n;
+  const int* pv = ipiv + (long long)b * n;
+  for (int j = 0; j < nb; ++j) {
+    const int r = k0 + j;
+    const int p = pv[r];
+    if (p != r) {
+      cd t = A[r];
+      A[r] = A[p];
+      A[p] = t;
+    }
+  }
+}
+
+// X <- L11^{-1} X for the nb x ncols block of rows [k0, k0+nb), columns
+// [c0, c0+ncols) of X (ld = n), L11 unit lower at (k0,k0) of L (ld = n).
+template <bool UNIT_LOWER>
+__global__ __launch_bounds__(256) void k_trsm_block(int n, int k0, int nb, int c0, int ncols,
+                                                    const cd* const* Lp, cd* const* Xp) {
+  __shared__ cd sT[NB][NB];
+  const int b = blockIdx.y;
+  const cd* __restrict__ Lm = Lp[b];
+  cd* __restrict__ X = Xp[b];
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    const int i = e % nb, j = e / nb;
+    sT[i][j] = Lm[(k0 + i) + (long long)(k0 + j) * n];
+  }
+  __syncthreads();
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncols) return;
+  cd* __restrict__ x = X + (long long)(c0 + col) * n + k0;
+  cd r[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) r[i] = (i < nb) ? x[i] : mk(0.0, 0.0);
+  if (UNIT_LOWER) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      if (i < nb) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+          if (k < i) r[i] -= sT[i][k] * r[k];
+      }
+    }
+  } else {  // upper, non-unit
+#pragma unroll
+    for (int ii = NB - 1; ii >= 0; --ii) {
+      if (ii < nb) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+          if (k > ii && k < nb) r[ii] -= sT[ii][k] * r[k];
+        cd d = sT[ii][ii];
+        if (d.re != 0.0 || d.im != 0.0) r[ii] = cdiv(r[ii], d);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    if (i < nb) x[i] = r[i];
+}
+
+void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,
+                    const int* info_map, int batch) {
+  if (n <= 0 || batch <= 0) return;
+  const cd* const* Ac = (const cd* const*)A;
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = (n - k0 < NB) ? (n - k0) : NB;
+    hipLaunchKernelGGL(k_getf2_panel, dim3(batch), dim3(256), 0, s, n, k0, nb, A, ipiv, info,
+                       info_map);
+    // swaps on the columns left and right of the panel
+    if (k0 > 0)
+      hipLaunchKernelGGL(k_laswp, dim3((k0 + 127) / 128, batch), dim3(128), 0, s, n, n, k0, nb, 0,
+                         k0, A, (const int*)ipiv);
+    const int rest = n - k0 - nb;
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_laswp, dim3((rest + 127) / 128, batch), dim3(128), 0, s, n, n, k0, nb,
+                         k0 + nb, n, A, (const int*)ipiv);
+      hipLaunchKernelGGL(k_trsm_block<true>, dim3((rest + 255) / 256, batch), dim3(256), 0, s, n,
+                         k0, nb, k0 + nb, rest, Ac, A);
+      // A22 -= L21 * U12
+      zgemm_batched(s, rest, rest, nb, mk(-1.0, 0.0), Ac, (long long)(k0 + nb) + (long long)k0 * n,
+                    1, n, 0, Ac, (long long)k0 + (long long)(k0 + nb) * n, 1, n, 0, mk(1.0, 0.0), A,
+                    (long long)(k0 + nb) + (long long)(k0 + nb) * n, 1, n, batch);
+    }
+  }
+}
+
+void zgetrs_batched(hipStream_t s, int n, int nrhs, const cd* const* LU, const int* ipiv,
+                    cd* const* B, int batch) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+  const cd* const* Bc = (const cd* const*)B;
+  // all row swaps, in order
+  hipLaunchKernelGGL(k_laswp, dim3((nrhs + 127) / 128, batch), dim3(128), 0, s, n, nrhs, 0, n, 0,
+                     nrhs, B, ipiv);
+  // forward substitution with unit L
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = (n - k0 < NB) ? (n - k0) : NB;
+    hipLaunchKernelGGL(k_trsm_block<true>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, s, n, k0,
+                       nb, 0, nrhs, LU, B);
+    const int rest = n - k0 - nb;
+    if (rest > 0)
+      zgemm_batched(s, rest, nrhs, nb, mk(-1.0, 0.0), LU, (long long)(k0 + nb) + (long long)k0 * n,
+                    1, n, 0, Bc, (long long)k0, 1, n, 0, mk(1.0, 0.0), B, (long long)(k0 + nb), 1,
+                    n, batch);
+  }
+  // backward substitution with U
+  const int nblk = (n + NB - 1) / NB;
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int k0 = bk * NB;
+    const int nb = (n - k0 < NB) ? (n - k0) : NB;
+    hipLaunchKernelGGL(k_trsm_block<false>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, s, n,
+                       k0, nb, 0, nrhs, LU, B);
+    if (k0 > 0)
+      zgemm_batched(s, k0, nrhs, nb, mk(-1.0, 0.0), LU, (long long)k0 * n, 1, n, 0, Bc,
+                    (long long)k0, 1, n, 0, mk(1.0, 0.0), B, 0, 1, n, batch);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise
+// ---------------------------------------------------------------------------
+__global__ void k_set_identity(int n, cd* const* Ap) {
+  cd* A = Ap[blockIdx.y];
+  const long long nn = (long long)n * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nn;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long i = e % n, j = e / n;
+    A[e] = (i == j) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+  }
+}
+
+__global__ void k_axpby(long long count, cd a, const cd* const* Ap, cd bb, const cd* const* Bp,
+                        cd* const* Cp) {
+  const cd* A = Ap[blockIdx.y];
+  const cd* B = Bp[blockIdx.y];
+  cd* C = Cp[blockIdx.y];
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x)
+    C[e] = a * A[e] + bb * B[e];
+}
+
+__global__ void k_copy(long long count, const cd* const* Ap, cd* const* Bp) {
+  const cd* A = Ap[blockIdx.y];
+  cd* B = Bp[blockIdx.y];
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x)
+    B[e] = A[e];
+}
+
+static inline int grid1d(long long count) {
+  long long g = (count + 255) / 256;
+  return (int)(g > 1024 ? 1024 : (g < 1 ? 1 : g));
+}
+
+void set_identity_batched(hipStream_t s, int n, cd* const* A, int batch) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_set_identity, dim3(grid1d((long long)n * n), batch), dim3(256), 0, s, n, A);
+}
+
+void axpby_batched(hipStream_t s, int n, cd a, const cd* const* A, cd b, const cd* const* B,
+                   cd* const* C, int batch) {
+  if (batch <= 0) return;
+  const long long cnt = (long long)n * n;
+  hipLaunchKernelGGL(k_axpby, dim3(grid1d(cnt), batch), dim3(256), 0, s, cnt, a, A, b, B, C);
+}
+
+void copy_batched(hipStream_t s, long long count, const cd* const* A, cd* const* B, int batch) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3(grid1d(count), batch), dim3(256), 0, s, count, A, B);
+}
+
+}  // namespace orcwa

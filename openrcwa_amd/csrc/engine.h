@@ -1,0 +1,56 @@
+// Internal interfaces of the OpenRCWA MI355X engine (host-side launchers of
+// the batched kernels).  Every launcher enqueues work on `s` and returns
+// immediately.  Batched operands are device arrays of base pointers (one per
+// batch entry) plus an element offset applied to every entry, so that
+// sub-blocks of a batch of matrices can be addressed without new tables.
+#pragma once
+
+#include "orcwa_common.h"
+#include "../../include/openrcwa_gpu.h"
+
+namespace orcwa {
+
+// Batched complex GEMM on FP64 MFMA:
+//   C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]
+// op(X)[i,k] = X[off + i*rs + k*cs], conjugated when conj != 0.
+// beta == 0 does not read C.
+void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
+                   const cd* const* A, long long offA, long long rsA, long long csA, int conjA,
+                   const cd* const* B, long long offB, long long rsB, long long csB, int conjB,
+                   cd beta, cd* const* C, long long offC, long long rsC, long long csC,
+                   int batch);
+
+// Batched LU with partial pivoting of n x n column-major matrices (lda = n),
+// in place.  ipiv[b*n + i] = global row swapped with row i.  info[b] gets
+// ORCWA_STATUS_SINGULAR or-ed in when an exactly zero pivot is met
+// (info[info_map[b]] when info_map is not null).
+void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,
+                    const int* info_map, int batch);
+
+// Solve A X = B in place of B (n x nrhs, column-major, ldb = n) with the
+// factors of zgetrf_batched.
+void zgetrs_batched(hipStream_t s, int n, int nrhs, const cd* const* LU, const int* ipiv,
+                    cd* const* B, int batch);
+
+// Elementwise helpers over batches of n x n column-major matrices.
+void set_identity_batched(hipStream_t s, int n, cd* const* A, int batch);
+// C = a*A + b*B (+ d*I when addI)
+void axpby_batched(hipStream_t s, int n, cd a, const cd* const* A, cd b, const cd* const* B,
+                   cd* const* C, int batch);
+void copy_batched(hipStream_t s, long long count, const cd* const* A, cd* const* B, int batch);
+
+// Eigensolver for a batch of general complex n x n matrices (column-major):
+// A[b] is overwritten (Schur form).  On exit W[b] holds unit-2-norm
+// eigenvectors and lam[b*n + j] the eigenvalues.  work must hold 2 n*n per
+// matrix (Z and X).  Aorig (may equal nullptr) is used for the residual test;
+// status bits are or-ed into info[b].  resid[b] receives the relative residual.
+struct EigWorkspace {
+  cd** Z;        // [batch] n*n
+  cd** X;        // [batch] n*n
+  cd* tau;       // [batch*n]
+};
+void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, cd* const* W,
+                   cd* lam, const EigWorkspace& ws, int* info, double* resid,
+                   cd* const* scratch, int batch, hipEvent_t* phase_events);
+
+}  // namespace orcwa

@@ -1,0 +1,920 @@
+// Batched layer-stack solve: plan construction, execution and the C ABI of
+// include/openrcwa_gpu.h.
+//
+// Pipeline of one execute (every step batched over instances, on one stream):
+//   1. convolution matrices of every distinct permittivity grid
+//      (FourierSolver2D::solve x3, rcwa/Layer.cpp:99-116), fe33^-1 and the
+//      four instance-independent products that P*Q needs
+//   2. per chunk of instances:
+//      a. P*Q of every patterned layer (rcwa/Layer.cpp:150-220)
+//      b. eigen-decomposition (rcwa/WaveEquationSolver.cpp:24-37)
+//      c. keff, V = Q W diag(1/keff); homogeneous layers in closed form
+//      d. interfaces E = W_a^-1 W_b, H = V_a^-1 V_b (rcwa/RCWASolver.cpp:495-569)
+//      e. RTCM cascade from the exit side (rcwa/RCWASolver.cpp:378-492)
+//      f. plane-wave response and per-order R/T (rcwa/RCWASolver.cpp:1778-1908)
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "context.h"
+#include "rcwa_kernels.h"
+
+using namespace orcwa;
+
+namespace {
+
+constexpr double kPi = 3.1415926535897932;  // rcwa/defns.h:25
+constexpr int kMaxPol = 8;
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// host-side image of a device blob of tables; resolved after allocation
+struct Blob {
+  std::vector<char> host;
+  template <class T>
+  size_t put(const std::vector<T>& v) {
+    size_t off = align256(host.size());
+    host.resize(off + v.size() * sizeof(T) + 8);
+    if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+struct StepGroup {
+  int count = 0;                 // instances in the group
+  // table offsets into the blob
+  size_t Wa = 0, Va = 0, Wb = 0, Vb = 0, binv_a = 0, E = 0, H = 0, LU = 0, inst = 0;
+};
+
+struct StepTables {
+  StepGroup ua, pa;              // a homogeneous / a patterned
+  size_t keff_b = 0, k0d = 0;    // per instance of the chunk
+};
+
+struct ChunkTables {
+  int i0 = 0, ninst = 0, nprob = 0, nuni = 0;
+  size_t prob_params = 0, inst_of_prob = 0, uni_params = 0, rt_params = 0;
+  // patterned problem slots
+  size_t pA = 0, pAorig = 0, pW = 0, pV = 0, pZ = 0, pX = 0, pQW = 0;
+  size_t pE11 = 0, pE22 = 0;     // grid convolution matrices of each problem
+  size_t uV = 0;                 // homogeneous V slots
+  // per instance RTCM slots
+  size_t R = 0, T = 0, E = 0, H = 0, F = 0, G = 0, LU = 0, Tau = 0;
+  long long prob_base = 0;       // global index of the chunk's first problem
+  std::vector<StepTables> steps;
+};
+
+}  // namespace
+
+struct orcwa_plan {
+  orcwa_gpu_context* ctx = nullptr;
+  int Nx = 0, Ny = 0, nx = 0, ny = 0, h = 0, n = 0, L = 0, count = 0;
+  int ncx = 0, ncy = 0, ncell = 0, maxpol = 1;
+  double Lx = 0, Ly = 0;
+  std::vector<double> cx, cy, thick;
+  std::vector<double> lam, kx0, ky0;
+  std::vector<int> npol;
+  std::vector<cd> pol;             // [count][8][2]
+  std::vector<int> gid;            // [count*L]: grid index, -1 homogeneous
+  std::vector<cd> ueps;            // [count*L]: value of a homogeneous layer
+  std::vector<cd> eps_ref, eps_trn;
+  int G = 0;
+  std::vector<cd> grid_vals;       // [G*ncell]
+  std::vector<int> prob_inst;      // global problem -> instance
+  long long nprob_total = 0;
+
+  int chunk = 0, nchunks = 0, pmax = 0, umax = 0;
+  std::vector<ChunkTables> chunks;
+
+  // device memory
+  char* d_ws = nullptr;            // workspace arena
+  char* d_blob = nullptr;          // tables
+  Blob blob;
+  // fixed device regions (pointers inside d_ws)
+  double *d_cx = nullptr, *d_cy = nullptr;
+  cd* d_grids = nullptr;
+  cd* d_strips = nullptr;
+  StripTask* d_tasks = nullptr;
+  long long* d_strip_base = nullptr;
+  int ntasks = 0;
+  size_t strip_lds = 0;
+  cd* d_ident = nullptr;           // one n x n identity
+  // grid matrices: E11, E22, E33, LU, M, S1..S4, tmp  (10 per grid)
+  std::vector<cd*> gE11, gE22, gE33, gLU, gM, gS1, gS2, gS3, gS4, gTmp;
+  size_t t_gE11 = 0, t_gE22 = 0, t_gE33 = 0, t_gLU = 0, t_gM = 0, t_gS1 = 0, t_gS2 = 0,
+         t_gS3 = 0, t_gS4 = 0, t_gTmp = 0, t_gridmats = 0;
+  int* d_gipiv = nullptr;
+  int* d_ginfo = nullptr;
+  // per chunk workspace (shared by all chunks)
+  cd* d_lam = nullptr;             // [pmax*n]
+  cd* d_keffp = nullptr;           // [pmax*n]
+  cd* d_keffu = nullptr;           // [umax*n]
+  cd* d_binv = nullptr;            // [umax*4h]
+  cd* d_tau = nullptr;             // [pmax*n]
+  int* d_ipiv = nullptr;           // [chunk*n]
+  int* d_pinfo = nullptr;          // [nprob_total]
+  double* d_resid = nullptr;       // [nprob_total]
+  // outputs
+  double* d_ref = nullptr;         // [count][maxpol][h]
+  double* d_trn = nullptr;
+  cd* d_amp = nullptr;             // [count][8][h]
+  int* d_info = nullptr;           // [count]
+
+  hipEvent_t ev[8] = {};
+  bool have_events = false;
+  double phase_ms[6] = {-1, -1, -1, -1, -1, -1};
+  double eig_ms_acc = 0.0;
+
+  template <class T>
+  T dev(size_t off) const {
+    return reinterpret_cast<T>(d_blob + off);
+  }
+};
+
+namespace {
+
+int fail(const char* msg) {
+  fprintf(stderr, "[openrcwa] %s\n", msg);
+  return ORCWA_ERR_ARG;
+}
+
+ConvGeom geom_of(const orcwa_plan* p) {
+  ConvGeom g;
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.ncx = p->ncx;
+  g.ncy = p->ncy;
+  g.ncell = p->ncell;
+  g.Lx = p->Lx;
+  g.Ly = p->Ly;
+  g.cx = p->d_cx;
+  g.cy = p->d_cy;
+  return g;
+}
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) {                                                                  \
+      fprintf(stderr, "[openrcwa] HIP error %s at %s:%d: %s\n", hipGetErrorName(_e), __FILE__, \
+              __LINE__, #expr);                                                              \
+      return ORCWA_ERR_HIP;                                                                  \
+    }                                                                                        \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int orcwa_gpu_create(orcwa_gpu_context** ctx, int device) {
+  if (!ctx) return fail("orcwa_gpu_create: null context pointer");
+  *ctx = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    fprintf(stderr, "[openrcwa] no HIP device available\n");
+    return ORCWA_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) return fail("orcwa_gpu_create: device index out of range");
+  HIPCHK(hipSetDevice(device));
+  orcwa_gpu_context* c = new orcwa_gpu_context;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    fprintf(stderr, "[openrcwa] hipStreamCreate failed\n");
+    return ORCWA_ERR_HIP;
+  }
+  *ctx = c;
+  return ORCWA_OK;
+}
+
+int orcwa_gpu_destroy(orcwa_gpu_context* ctx) {
+  if (!ctx) return ORCWA_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ORCWA_OK;
+}
+
+int orcwa_plan_destroy(orcwa_plan* p) {
+  if (!p) return ORCWA_OK;
+  hipSetDevice(p->ctx->device);
+  if (p->d_ws) hipFree(p->d_ws);
+  if (p->d_blob) hipFree(p->d_blob);
+  if (p->have_events)
+    for (auto& e : p->ev) hipEventDestroy(e);
+  delete p;
+  return ORCWA_OK;
+}
+
+int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
+                      int count, orcwa_plan** out) {
+  if (!out) return fail("orcwa_plan_create: null plan pointer");
+  *out = nullptr;
+  if (!ctx || !st || !inst) return fail("orcwa_plan_create: null argument");
+  if (count <= 0) return fail("orcwa_plan_create: count must be positive");
+  if (st->nx_harm < 0 || st->ny_harm < 0) return fail("negative harmonic count");
+  if (st->n_cx < 2 || st->n_cy < 2 || !st->coord_x || !st->coord_y)
+    return fail("each axis needs at least two cell edges");
+  if (st->n_layers < 2) return fail("a stack needs at least 2 layers (incidence and exit)");
+  if (!st->thickness) return fail("null thickness array");
+  const int nx = 2 * st->nx_harm + 1, ny = 2 * st->ny_harm + 1;
+  if (nx > 63 || ny > 63) return fail("at most 31 harmonics per axis are supported");
+  for (int i = 1; i < st->n_cx; ++i)
+    if (!(st->coord_x[i] > st->coord_x[i - 1])) return fail("coord_x must be strictly increasing");
+  for (int i = 1; i < st->n_cy; ++i)
+    if (!(st->coord_y[i] > st->coord_y[i - 1])) return fail("coord_y must be strictly increasing");
+
+  hipSetDevice(ctx->device);
+  orcwa_plan* p = new orcwa_plan;
+  p->ctx = ctx;
+  p->Nx = st->nx_harm;
+  p->Ny = st->ny_harm;
+  p->nx = nx;
+  p->ny = ny;
+  p->h = nx * ny;
+  p->n = 2 * p->h;
+  p->L = st->n_layers;
+  p->count = count;
+  p->ncx = st->n_cx;
+  p->ncy = st->n_cy;
+  p->ncell = (p->ncx - 1) * (p->ncy - 1);
+  p->cx.assign(st->coord_x, st->coord_x + p->ncx);
+  p->cy.assign(st->coord_y, st->coord_y + p->ncy);
+  p->Lx = p->cx.back() - p->cx.front();
+  p->Ly = p->cy.back() - p->cy.front();
+  p->thick.assign(st->thickness, st->thickness + p->L);
+  const int L = p->L, h = p->h, n = p->n, ncell = p->ncell;
+
+  // ---- instances: classify layers, dedupe grids
+  std::map<std::vector<double>, int> grid_index;
+  p->lam.resize(count);
+  p->kx0.resize(count);
+  p->ky0.resize(count);
+  p->npol.resize(count);
+  p->pol.assign((size_t)count * kMaxPol * 2, mk(0, 0));
+  p->gid.assign((size_t)count * L, -1);
+  p->ueps.assign((size_t)count * L, mk(0, 0));
+  p->eps_ref.resize(count);
+  p->eps_trn.resize(count);
+  for (int i = 0; i < count; ++i) {
+    const orcwa_instance& I = inst[i];
+    if (!(I.lambda > 0) || !std::isfinite(I.lambda)) {
+      delete p;
+      return fail("instance wavelength must be positive");
+    }
+    if (!I.eps || I.n_pol < 1 || I.n_pol > kMaxPol || !I.pol) {
+      delete p;
+      return fail("instance needs eps and 1..8 polarisations");
+    }
+    p->lam[i] = I.lambda;
+    p->kx0[i] = I.kx0;
+    p->ky0[i] = I.ky0;
+    p->npol[i] = I.n_pol;
+    p->maxpol = std::max(p->maxpol, I.n_pol);
+    for (int q = 0; q < I.n_pol; ++q) {
+      p->pol[((size_t)i * kMaxPol + q) * 2 + 0] = mk(I.pol[4 * q + 0], I.pol[4 * q + 1]);
+      p->pol[((size_t)i * kMaxPol + q) * 2 + 1] = mk(I.pol[4 * q + 2], I.pol[4 * q + 3]);
+    }
+    for (int l = 0; l < L; ++l) {
+      const double* e = I.eps + (size_t)l * ncell * 2;
+      bool uniform = true;
+      for (int c = 1; c < ncell && uniform; ++c)
+        if (e[2 * c] != e[0] || e[2 * c + 1] != e[1]) uniform = false;
+      if (uniform) {
+        p->ueps[(size_t)i * L + l] = mk(e[0], e[1]);
+        if (e[0] == 0.0 && e[1] == 0.0) {
+          delete p;
+          return fail("zero permittivity");
+        }
+      } else {
+        for (int c = 0; c < ncell; ++c)
+          if (e[2 * c] == 0.0 && e[2 * c + 1] == 0.0) {
+            delete p;
+            return fail("zero permittivity");
+          }
+        std::vector<double> key(e, e + 2 * ncell);
+        auto it = grid_index.find(key);
+        int g;
+        if (it == grid_index.end()) {
+          g = (int)grid_index.size();
+          grid_index.emplace(std::move(key), g);
+          for (int c = 0; c < ncell; ++c) p->grid_vals.push_back(mk(e[2 * c], e[2 * c + 1]));
+        } else {
+          g = it->second;
+        }
+        p->gid[(size_t)i * L + l] = g;
+      }
+    }
+    if (p->gid[(size_t)i * L] >= 0 || p->gid[(size_t)i * L + L - 1] >= 0) {
+      delete p;
+      fprintf(stderr,
+              "[openrcwa] the incidence and exit half-spaces must be homogeneous (instance %d)\n",
+              i);
+      return ORCWA_ERR_UNSUPPORTED;
+    }
+    p->eps_ref[i] = p->ueps[(size_t)i * L];
+    p->eps_trn[i] = p->ueps[(size_t)i * L + L - 1];
+  }
+  p->G = (int)grid_index.size();
+
+  // ---- chunking by a workspace budget
+  const size_t slot = (size_t)n * n * sizeof(cd);
+  int lp_max = 0, lu_max = 0;
+  for (int i = 0; i < count; ++i) {
+    int a = 0, b = 0;
+    for (int l = 0; l < L; ++l) (p->gid[(size_t)i * L + l] >= 0 ? a : b)++;
+    lp_max = std::max(lp_max, a);
+    lu_max = std::max(lu_max, b);
+  }
+  double budget_gb = 32.0;
+  if (const char* s = getenv("ORCWA_WORKSPACE_GB")) budget_gb = atof(s);
+  const size_t per_inst = (size_t)(7 * lp_max + lu_max + 8) * slot;
+  long long c_fit = (long long)(budget_gb * 1e9 / (double)per_inst);
+  int chunk = (int)std::max(1LL, std::min((long long)count, c_fit));
+  if (const char* s = getenv("ORCWA_CHUNK")) chunk = std::max(1, std::min(count, atoi(s)));
+  p->chunk = chunk;
+  p->nchunks = (count + chunk - 1) / chunk;
+
+  // ---- problems per chunk
+  p->pmax = 0;
+  p->umax = 0;
+  for (int c = 0; c < p->nchunks; ++c) {
+    const int i0 = c * chunk, i1 = std::min(count, i0 + chunk);
+    int np = 0, nu = 0;
+    for (int i = i0; i < i1; ++i)
+      for (int l = 0; l < L; ++l) (p->gid[(size_t)i * L + l] >= 0 ? np : nu)++;
+    p->pmax = std::max(p->pmax, np);
+    p->umax = std::max(p->umax, nu);
+  }
+
+  // ---- workspace layout
+  const int G = p->G;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = align256(off);
+    off = o + bytes;
+    return o;
+  };
+  const size_t o_cx = take(p->ncx * sizeof(double));
+  const size_t o_cy = take(p->ncy * sizeof(double));
+  const size_t o_grids = take((size_t)std::max(1, G) * ncell * sizeof(cd));
+  // strips: type-0 records per grid (ncy-1 of nx^2+My+Mx), type-1 (ncx-1 of ny^2+Mx)
+  const long long rec0 = (long long)nx * nx + (2 * ny - 1) + (2 * nx - 1);
+  const long long rec1 = (long long)ny * ny + (2 * nx - 1);
+  const long long per_grid_strip = (p->ncy - 1) * rec0 + (p->ncx - 1) * rec1;
+  const size_t o_strips = take((size_t)std::max(1, G) * per_grid_strip * sizeof(cd));
+  std::vector<StripTask> tasks;
+  std::vector<long long> strip_base;
+  for (int g = 0; g < G; ++g) {
+    long long b0 = g * per_grid_strip, b1 = b0 + (p->ncy - 1) * rec0;
+    strip_base.push_back(b0);
+    strip_base.push_back(b1);
+    for (int i = 0; i < p->ncy - 1; ++i) tasks.push_back({g, 0, i, 0, b0 + i * rec0});
+    for (int i = 0; i < p->ncx - 1; ++i) tasks.push_back({g, 1, i, 0, b1 + i * rec1});
+  }
+  p->ntasks = (int)tasks.size();
+  const int mdim = std::max(nx, ny);
+  p->strip_lds = (size_t)(2 * mdim - 1 + mdim * mdim + std::max(p->ncx, p->ncy)) * sizeof(cd);
+  const size_t o_tasks = take(std::max<size_t>(1, tasks.size()) * sizeof(StripTask));
+  const size_t o_sbase = take(std::max<size_t>(1, strip_base.size()) * sizeof(long long));
+  const size_t o_ident = take(slot);
+  const size_t gslot = (size_t)h * h * sizeof(cd);
+  const size_t o_gm = take((size_t)std::max(1, G) * 10 * gslot);
+  const size_t o_gipiv = take((size_t)std::max(1, G) * h * sizeof(int));
+  const size_t o_ginfo = take((size_t)std::max(1, G) * sizeof(int));
+  const int PM = std::max(1, p->pmax), UM = std::max(1, p->umax);
+  const size_t o_pslots = take((size_t)PM * 7 * slot);
+  const size_t o_uslots = take((size_t)UM * slot);
+  const size_t o_islots = take((size_t)chunk * 8 * slot);
+  const size_t o_lam = take((size_t)PM * n * sizeof(cd));
+  const size_t o_keffp = take((size_t)PM * n * sizeof(cd));
+  const size_t o_keffu = take((size_t)UM * n * sizeof(cd));
+  const size_t o_binv = take((size_t)UM * 4 * h * sizeof(cd));
+  const size_t o_tau = take((size_t)PM * n * sizeof(cd));
+  const size_t o_ipiv = take((size_t)std::max(chunk, PM) * n * sizeof(int));
+  p->nprob_total = 0;
+  for (int i = 0; i < count; ++i)
+    for (int l = 0; l < L; ++l)
+      if (p->gid[(size_t)i * L + l] >= 0) {
+        p->prob_inst.push_back(i);
+        ++p->nprob_total;
+      }
+  const size_t o_pinfo = take((size_t)std::max<long long>(1, p->nprob_total) * sizeof(int));
+  const size_t o_resid = take((size_t)std::max<long long>(1, p->nprob_total) * sizeof(double));
+  const size_t o_ref = take((size_t)count * p->maxpol * h * sizeof(double));
+  const size_t o_trn = take((size_t)count * p->maxpol * h * sizeof(double));
+  const size_t o_amp = take((size_t)count * 8 * h * sizeof(cd));
+  const size_t o_info = take((size_t)count * sizeof(int));
+  const size_t ws_bytes = align256(off);
+
+  if (hipMalloc(&p->d_ws, ws_bytes) != hipSuccess) {
+    delete p;
+    fprintf(stderr, "[openrcwa] could not allocate %.2f GB of device workspace\n", ws_bytes / 1e9);
+    return ORCWA_ERR_NOMEM;
+  }
+  char* W = p->d_ws;
+  p->d_cx = (double*)(W + o_cx);
+  p->d_cy = (double*)(W + o_cy);
+  p->d_grids = (cd*)(W + o_grids);
+  p->d_strips = (cd*)(W + o_strips);
+  p->d_tasks = (StripTask*)(W + o_tasks);
+  p->d_strip_base = (long long*)(W + o_sbase);
+  p->d_ident = (cd*)(W + o_ident);
+  p->d_gipiv = (int*)(W + o_gipiv);
+  p->d_ginfo = (int*)(W + o_ginfo);
+  p->d_lam = (cd*)(W + o_lam);
+  p->d_keffp = (cd*)(W + o_keffp);
+  p->d_keffu = (cd*)(W + o_keffu);
+  p->d_binv = (cd*)(W + o_binv);
+  p->d_tau = (cd*)(W + o_tau);
+  p->d_ipiv = (int*)(W + o_ipiv);
+  p->d_pinfo = (int*)(W + o_pinfo);
+  p->d_resid = (double*)(W + o_resid);
+  p->d_ref = (double*)(W + o_ref);
+  p->d_trn = (double*)(W + o_trn);
+  p->d_amp = (cd*)(W + o_amp);
+  p->d_info = (int*)(W + o_info);
+
+  // constant inputs
+  hipStream_t s = ctx->stream;
+  HIPCHK(hipMemcpyAsync(p->d_cx, p->cx.data(), p->ncx * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p->d_cy, p->cy.data(), p->ncy * sizeof(double), hipMemcpyHostToDevice, s));
+  if (G > 0) {
+    HIPCHK(hipMemcpyAsync(p->d_grids, p->grid_vals.data(), (size_t)G * ncell * sizeof(cd),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(p->d_tasks, tasks.data(), tasks.size() * sizeof(StripTask),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(p->d_strip_base, strip_base.data(), strip_base.size() * sizeof(long long),
+                          hipMemcpyHostToDevice, s));
+  }
+
+  // ---- tables
+  Blob& B = p->blob;
+  auto gmat = [&](int g, int k) { return (cd*)(W + o_gm + ((size_t)g * 10 + k) * gslot); };
+  for (int g = 0; g < G; ++g) {
+    p->gE11.push_back(gmat(g, 0));
+    p->gE22.push_back(gmat(g, 1));
+    p->gE33.push_back(gmat(g, 2));
+    p->gLU.push_back(gmat(g, 3));
+    p->gM.push_back(gmat(g, 4));
+    p->gS1.push_back(gmat(g, 5));
+    p->gS2.push_back(gmat(g, 6));
+    p->gS3.push_back(gmat(g, 7));
+    p->gS4.push_back(gmat(g, 8));
+    p->gTmp.push_back(gmat(g, 9));
+  }
+  p->t_gE11 = B.put(p->gE11);
+  p->t_gE22 = B.put(p->gE22);
+  p->t_gE33 = B.put(p->gE33);
+  p->t_gLU = B.put(p->gLU);
+  p->t_gM = B.put(p->gM);
+  p->t_gS1 = B.put(p->gS1);
+  p->t_gS2 = B.put(p->gS2);
+  p->t_gS3 = B.put(p->gS3);
+  p->t_gS4 = B.put(p->gS4);
+  p->t_gTmp = B.put(p->gTmp);
+  {
+    std::vector<GridMats> gm(G);
+    for (int g = 0; g < G; ++g)
+      gm[g] = {p->gE11[g], p->gE22[g], p->gS1[g], p->gS2[g], p->gS3[g], p->gS4[g]};
+    p->t_gridmats = B.put(gm);
+  }
+
+  auto pslot = [&](int pi, int k) { return (cd*)(W + o_pslots + ((size_t)pi * 7 + k) * slot); };
+  auto uslot = [&](int ui) { return (cd*)(W + o_uslots + (size_t)ui * slot); };
+  auto islot = [&](int ii, int k) { return (cd*)(W + o_islots + ((size_t)ii * 8 + k) * slot); };
+
+  long long prob_global = 0;
+  for (int c = 0; c < p->nchunks; ++c) {
+    ChunkTables ct;
+    ct.i0 = c * chunk;
+    ct.ninst = std::min(count, ct.i0 + chunk) - ct.i0;
+    ct.prob_base = prob_global;
+    // (instance, layer) -> problem index or homogeneous index
+    std::vector<int> pidx((size_t)ct.ninst * L, -1), uidx((size_t)ct.ninst * L, -1);
+    std::vector<ProbParam> pp;
+    std::vector<int> inst_of;
+    std::vector<UniParam> up;
+    for (int ii = 0; ii < ct.ninst; ++ii) {
+      const int i = ct.i0 + ii;
+      const double inv_k0 = p->lam[i] / (2.0 * kPi);
+      for (int l = 0; l < L; ++l) {
+        const int g = p->gid[(size_t)i * L + l];
+        if (g >= 0) {
+          pidx[(size_t)ii * L + l] = (int)pp.size();
+          ProbParam q;
+          q.kx0 = p->kx0[i];
+          q.ky0 = p->ky0[i];
+          q.inv_k0 = inv_k0;
+          q.grid = g;
+          q.pad = 0;
+          pp.push_back(q);
+          inst_of.push_back(i);
+        } else {
+          uidx[(size_t)ii * L + l] = (int)up.size();
+          UniParam u;
+          u.eps = p->ueps[(size_t)i * L + l];
+          u.kx0 = p->kx0[i];
+          u.ky0 = p->ky0[i];
+          u.inv_k0 = inv_k0;
+          u.inst = i;
+          u.pad = 0;
+          up.push_back(u);
+        }
+      }
+    }
+    ct.nprob = (int)pp.size();
+    ct.nuni = (int)up.size();
+    prob_global += ct.nprob;
+    ct.prob_params = B.put(pp);
+    ct.inst_of_prob = B.put(inst_of);
+    ct.uni_params = B.put(up);
+    std::vector<cd*> tA, tAo, tW, tV, tZ, tX, tQW, tUV;
+    for (int pi = 0; pi < ct.nprob; ++pi) {
+      tA.push_back(pslot(pi, 0));
+      tAo.push_back(pslot(pi, 1));
+      tW.push_back(pslot(pi, 2));
+      tV.push_back(pslot(pi, 3));
+      tZ.push_back(pslot(pi, 4));
+      tX.push_back(pslot(pi, 5));
+      tQW.push_back(pslot(pi, 6));
+    }
+    for (int ui = 0; ui < ct.nuni; ++ui) tUV.push_back(uslot(ui));
+    ct.pA = B.put(tA);
+    ct.pAorig = B.put(tAo);
+    ct.pW = B.put(tW);
+    ct.pV = B.put(tV);
+    ct.pZ = B.put(tZ);
+    ct.pX = B.put(tX);
+    ct.pQW = B.put(tQW);
+    ct.uV = B.put(tUV);
+    {
+      std::vector<cd*> e11, e22;
+      for (int pi = 0; pi < ct.nprob; ++pi) {
+        e11.push_back(p->gE11[pp[pi].grid]);
+        e22.push_back(p->gE22[pp[pi].grid]);
+      }
+      ct.pE11 = B.put(e11);
+      ct.pE22 = B.put(e22);
+    }
+    std::vector<cd*> tR, tT, tE, tH, tF, tG, tLU, tTau;
+    for (int ii = 0; ii < ct.ninst; ++ii) {
+      tR.push_back(islot(ii, 0));
+      tT.push_back(islot(ii, 1));
+      tE.push_back(islot(ii, 2));
+      tH.push_back(islot(ii, 3));
+      tF.push_back(islot(ii, 4));
+      tG.push_back(islot(ii, 5));
+      tLU.push_back(islot(ii, 6));
+      tTau.push_back(islot(ii, 7));
+    }
+    ct.R = B.put(tR);
+    ct.T = B.put(tT);
+    ct.E = B.put(tE);
+    ct.H = B.put(tH);
+    ct.F = B.put(tF);
+    ct.G = B.put(tG);
+    ct.LU = B.put(tLU);
+    ct.Tau = B.put(tTau);
+    // layer references
+    auto Wof = [&](int ii, int l) -> cd* {
+      int q = pidx[(size_t)ii * L + l];
+      return q >= 0 ? pslot(q, 2) : p->d_ident;
+    };
+    auto Vof = [&](int ii, int l) -> cd* {
+      int q = pidx[(size_t)ii * L + l];
+      return q >= 0 ? pslot(q, 3) : uslot(uidx[(size_t)ii * L + l]);
+    };
+    auto Kof = [&](int ii, int l) -> cd* {
+      int q = pidx[(size_t)ii * L + l];
+      return q >= 0 ? p->d_keffp + (size_t)q * n : p->d_keffu + (size_t)uidx[(size_t)ii * L + l] * n;
+    };
+    auto Bof = [&](int ii, int l) -> cd* {
+      int u = uidx[(size_t)ii * L + l];
+      return u >= 0 ? p->d_binv + (size_t)u * 4 * h : nullptr;
+    };
+    for (int sidx = 0; sidx < L - 1; ++sidx) {
+      StepTables stt;
+      const int idx = L - 1 - sidx, a = idx - 1, b = idx;
+      std::vector<cd*> kb;
+      std::vector<double> k0d;
+      std::vector<cd*> uaWb, uaVb, uaBinv, uaE, uaH;
+      std::vector<cd*> paWa, paVa, paWb, paVb, paE, paH, paLU;
+      std::vector<int> uaInst, paInst;
+      for (int ii = 0; ii < ct.ninst; ++ii) {
+        const int i = ct.i0 + ii;
+        kb.push_back(Kof(ii, b));
+        k0d.push_back(2.0 * kPi / p->lam[i] * p->thick[b]);
+        if (pidx[(size_t)ii * L + a] < 0) {
+          uaWb.push_back(Wof(ii, b));
+          uaVb.push_back(Vof(ii, b));
+          uaBinv.push_back(Bof(ii, a));
+          uaE.push_back(islot(ii, 2));
+          uaH.push_back(islot(ii, 3));
+          uaInst.push_back(i);
+        } else {
+          paWa.push_back(Wof(ii, a));
+          paVa.push_back(Vof(ii, a));
+          paWb.push_back(Wof(ii, b));
+          paVb.push_back(Vof(ii, b));
+          paE.push_back(islot(ii, 2));
+          paH.push_back(islot(ii, 3));
+          paLU.push_back(islot(ii, 6));
+          paInst.push_back(i);
+        }
+      }
+      stt.keff_b = B.put(kb);
+      stt.k0d = B.put(k0d);
+      stt.ua.count = (int)uaInst.size();
+      stt.ua.Wb = B.put(uaWb);
+      stt.ua.Vb = B.put(uaVb);
+      stt.ua.binv_a = B.put(uaBinv);
+      stt.ua.E = B.put(uaE);
+      stt.ua.H = B.put(uaH);
+      stt.ua.inst = B.put(uaInst);
+      stt.pa.count = (int)paInst.size();
+      stt.pa.Wa = B.put(paWa);
+      stt.pa.Va = B.put(paVa);
+      stt.pa.Wb = B.put(paWb);
+      stt.pa.Vb = B.put(paVb);
+      stt.pa.E = B.put(paE);
+      stt.pa.H = B.put(paH);
+      stt.pa.LU = B.put(paLU);
+      stt.pa.inst = B.put(paInst);
+      ct.steps.push_back(stt);
+    }
+    std::vector<RtParam> rp(ct.ninst);
+    for (int ii = 0; ii < ct.ninst; ++ii) {
+      const int i = ct.i0 + ii;
+      RtParam& r = rp[ii];
+      std::memset(&r, 0, sizeof(r));
+      r.eps_ref = p->eps_ref[i];
+      r.eps_trn = p->eps_trn[i];
+      r.kx0 = p->kx0[i];
+      r.ky0 = p->ky0[i];
+      r.inv_k0 = p->lam[i] / (2.0 * kPi);
+      r.npol = p->npol[i];
+      for (int q = 0; q < kMaxPol; ++q) {
+        r.pol[q][0] = p->pol[((size_t)i * kMaxPol + q) * 2 + 0];
+        r.pol[q][1] = p->pol[((size_t)i * kMaxPol + q) * 2 + 1];
+      }
+    }
+    ct.rt_params = B.put(rp);
+    p->chunks.push_back(std::move(ct));
+  }
+  if (hipMalloc(&p->d_blob, std::max<size_t>(256, B.host.size())) != hipSuccess) {
+    orcwa_plan_destroy(p);
+    return ORCWA_ERR_NOMEM;
+  }
+  HIPCHK(hipMemcpyAsync(p->d_blob, B.host.data(), B.host.size(), hipMemcpyHostToDevice, s));
+  // identity matrix
+  {
+    std::vector<cd*> one{p->d_ident};
+    // small host->device table for the identity fill
+    cd** d_tab = nullptr;
+    HIPCHK(hipMalloc(&d_tab, sizeof(cd*)));
+    HIPCHK(hipMemcpyAsync(d_tab, one.data(), sizeof(cd*), hipMemcpyHostToDevice, s));
+    set_identity_batched(s, n, d_tab, 1);
+    HIPCHK(hipStreamSynchronize(s));
+    hipFree(d_tab);
+  }
+  for (auto& e : p->ev) HIPCHK(hipEventCreate(&e));
+  p->have_events = true;
+  HIPCHK(hipStreamSynchronize(s));
+  *out = p;
+  return ORCWA_OK;
+}
+
+int orcwa_plan_execute(orcwa_plan* p) {
+  if (!p) return fail("null plan");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  const int n = p->n, h = p->h, G = p->G;
+  const ConvGeom g = geom_of(p);
+  const cd one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const long long nn = (long long)n * n;
+
+  HIPCHK(hipMemsetAsync(p->d_info, 0, p->count * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(p->d_pinfo, 0, std::max<long long>(1, p->nprob_total) * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(p->d_ginfo, 0, std::max(1, G) * sizeof(int), s));
+  HIPCHK(hipEventRecord(p->ev[0], s));
+
+  // ---- 1. convolution matrices and per-grid products
+  if (G > 0) {
+    launch_conv_strip(s, g, p->d_tasks, p->ntasks, p->d_grids, p->d_strips, p->strip_lds);
+    launch_conv_expand(s, g, p->d_strips, p->d_strip_base, p->dev<cd* const*>(p->t_gE11),
+                       p->dev<cd* const*>(p->t_gE22), p->dev<cd* const*>(p->t_gE33), G);
+    copy_batched(s, (long long)h * h, p->dev<const cd* const*>(p->t_gE33),
+                 p->dev<cd* const*>(p->t_gLU), G);
+    zgetrf_batched(s, h, p->dev<cd* const*>(p->t_gLU), p->d_gipiv, p->d_ginfo, nullptr, G);
+    set_identity_batched(s, h, p->dev<cd* const*>(p->t_gM), G);
+    zgetrs_batched(s, h, h, p->dev<const cd* const*>(p->t_gLU), p->d_gipiv,
+                   p->dev<cd* const*>(p->t_gM), G);
+    const cd* const* Mt = p->dev<const cd* const*>(p->t_gM);
+    zgemm_batched(s, h, h, h, one, Mt, 0, 1, h, 0, p->dev<const cd* const*>(p->t_gE11), 0, 1, h,
+                  0, zero, p->dev<cd* const*>(p->t_gS1), 0, 1, h, G);
+    launch_scale_rows_order(s, g, 0, p->dev<const cd* const*>(p->t_gE11),
+                            p->dev<cd* const*>(p->t_gTmp), G);
+    zgemm_batched(s, h, h, h, one, Mt, 0, 1, h, 0, p->dev<const cd* const*>(p->t_gTmp), 0, 1, h,
+                  0, zero, p->dev<cd* const*>(p->t_gS2), 0, 1, h, G);
+    zgemm_batched(s, h, h, h, one, Mt, 0, 1, h, 0, p->dev<const cd* const*>(p->t_gE22), 0, 1, h,
+                  0, zero, p->dev<cd* const*>(p->t_gS3), 0, 1, h, G);
+    launch_scale_rows_order(s, g, 1, p->dev<const cd* const*>(p->t_gE22),
+                            p->dev<cd* const*>(p->t_gTmp), G);
+    zgemm_batched(s, h, h, h, one, Mt, 0, 1, h, 0, p->dev<const cd* const*>(p->t_gTmp), 0, 1, h,
+                  0, zero, p->dev<cd* const*>(p->t_gS4), 0, 1, h, G);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(p->ev[1], s));
+  double ms_modes = 0, ms_cascade = 0, ms_rt = 0, ms_eig = 0;
+
+  for (int c = 0; c < p->nchunks; ++c) {
+    const ChunkTables& ct = p->chunks[c];
+    const int np = ct.nprob, nu = ct.nuni, ni = ct.ninst;
+    HIPCHK(hipEventRecord(p->ev[2], s));
+    // ---- 2a-c. layer modes
+    if (np > 0) {
+      cd* const* A = p->dev<cd* const*>(ct.pA);
+      cd* const* Wp = p->dev<cd* const*>(ct.pW);
+      const cd* const* Wc = p->dev<const cd* const*>(ct.pW);
+      launch_build_pq(s, g, p->dev<const ProbParam*>(ct.prob_params),
+                      p->dev<const GridMats*>(p->t_gridmats), A, p->dev<cd* const*>(ct.pAorig),
+                      np);
+      EigWorkspace ws;
+      ws.Z = p->dev<cd**>(ct.pZ);
+      ws.X = p->dev<cd**>(ct.pX);
+      ws.tau = p->d_tau;
+      HIPCHK(hipEventRecord(p->ev[5], s));
+      zgeev_batched(s, n, A, p->dev<const cd* const*>(ct.pAorig), Wp, p->d_lam, ws,
+                    p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base,
+                    p->dev<cd* const*>(ct.pX), np, nullptr);
+      HIPCHK(hipEventRecord(p->ev[6], s));
+      // QW1 = [E22 W_bottom ; E11 W_top]
+      cd* const* QW = p->dev<cd* const*>(ct.pQW);
+      zgemm_batched(s, h, n, h, one, p->dev<const cd* const*>(ct.pE22), 0, 1, h, 0, Wc, h, 1, n, 0,
+                    zero, QW, 0, 1, n, np);
+      zgemm_batched(s, h, n, h, one, p->dev<const cd* const*>(ct.pE11), 0, 1, h, 0, Wc, 0, 1, n, 0,
+                    zero, QW, h, 1, n, np);
+      launch_build_v(s, g, p->dev<const ProbParam*>(ct.prob_params), Wc,
+                     p->dev<const cd* const*>(ct.pQW), p->d_lam, p->d_keffp,
+                     p->dev<cd* const*>(ct.pV), p->d_info, p->dev<const int*>(ct.inst_of_prob), np);
+    }
+    if (nu > 0)
+      launch_uniform(s, g, p->dev<const UniParam*>(ct.uni_params), p->d_keffu,
+                     p->dev<cd* const*>(ct.uV), p->d_binv, p->d_info, nu);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(p->ev[3], s));
+
+    // ---- 2d-e. interfaces and the RTCM cascade, exit side first
+    cd* const* R = p->dev<cd* const*>(ct.R);
+    cd* const* T = p->dev<cd* const*>(ct.T);
+    cd* const* E = p->dev<cd* const*>(ct.E);
+    cd* const* H = p->dev<cd* const*>(ct.H);
+    cd* const* F = p->dev<cd* const*>(ct.F);
+    cd* const* Gm = p->dev<cd* const*>(ct.G);
+    cd* const* LU = p->dev<cd* const*>(ct.LU);
+    cd* const* Tau = p->dev<cd* const*>(ct.Tau);
+    auto C = [](cd* const* x) { return (const cd* const*)x; };
+    launch_zero_mats(s, nn, R, ni);
+    set_identity_batched(s, n, T, ni);
+    for (int si = 0; si < p->L - 1; ++si) {
+      const StepTables& st = ct.steps[si];
+      // interfaces
+      if (st.ua.count > 0) {
+        const StepGroup& q = st.ua;
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Wb), p->dev<cd* const*>(q.E), q.count);
+        launch_bdinv_apply(s, h, p->dev<const cd* const*>(q.binv_a),
+                           p->dev<const cd* const*>(q.Vb), p->dev<cd* const*>(q.H), q.count);
+      }
+      if (st.pa.count > 0) {
+        const StepGroup& q = st.pa;
+        cd* const* qLU = p->dev<cd* const*>(q.LU);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Wa), qLU, q.count);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Wb), p->dev<cd* const*>(q.E), q.count);
+        zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
+        zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.E), q.count);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Va), qLU, q.count);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Vb), p->dev<cd* const*>(q.H), q.count);
+        zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
+        zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.H), q.count);
+      }
+      // cascade step
+      if (si > 0) {
+        launch_phase_scale(s, n, p->dev<const cd* const*>(st.keff_b), p->dev<const double*>(st.k0d),
+                           R, T, ni);
+        // F = E + E R~, G = H - H R~
+        copy_batched(s, nn, C(E), F, ni);
+        zgemm_batched(s, n, n, n, one, C(E), 0, 1, n, 0, C(R), 0, 1, n, 0, one, F, 0, 1, n, ni);
+        copy_batched(s, nn, C(H), Gm, ni);
+        zgemm_batched(s, n, n, n, mk(-1.0, 0.0), C(H), 0, 1, n, 0, C(R), 0, 1, n, 0, one, Gm, 0, 1,
+                      n, ni);
+      } else {
+        copy_batched(s, nn, C(E), F, ni);
+        copy_batched(s, nn, C(H), Gm, ni);
+      }
+      // tau = (F + G)^-1
+      axpby_batched(s, n, one, C(F), one, C(Gm), LU, ni);
+      zgetrf_batched(s, n, LU, p->d_ipiv, p->d_info + ct.i0, nullptr, ni);
+      set_identity_batched(s, n, Tau, ni);
+      zgetrs_batched(s, n, n, C(LU), p->d_ipiv, Tau, ni);
+      // R = I - 2 G tau ; T = 2 T~ tau
+      set_identity_batched(s, n, R, ni);
+      zgemm_batched(s, n, n, n, mk(-2.0, 0.0), C(Gm), 0, 1, n, 0, C(Tau), 0, 1, n, 0, one, R, 0, 1,
+                    n, ni);
+      zgemm_batched(s, n, n, n, mk(2.0, 0.0), C(T), 0, 1, n, 0, C(Tau), 0, 1, n, 0, zero, E, 0, 1,
+                    n, ni);
+      copy_batched(s, nn, C(E), T, ni);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(p->ev[4], s));
+    // ---- 2f. plane-wave response
+    launch_rt(s, g, p->dev<const RtParam*>(ct.rt_params), C(R), C(T),
+              p->d_ref + (size_t)ct.i0 * p->maxpol * h, p->d_trn + (size_t)ct.i0 * p->maxpol * h,
+              p->d_amp + (size_t)ct.i0 * 8 * h, p->d_info + ct.i0, p->maxpol, ni);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(p->ev[7], s));
+    if (p->nchunks > 1 || c == p->nchunks - 1) {
+      // per-phase timing of this chunk (waits for the chunk; chunks run in order)
+      HIPCHK(hipEventSynchronize(p->ev[7]));
+      float t = 0;
+      if (hipEventElapsedTime(&t, p->ev[2], p->ev[3]) == hipSuccess) ms_modes += t;
+      if (np > 0 && hipEventElapsedTime(&t, p->ev[5], p->ev[6]) == hipSuccess) ms_eig += t;
+      if (hipEventElapsedTime(&t, p->ev[3], p->ev[4]) == hipSuccess) ms_cascade += t;
+      if (hipEventElapsedTime(&t, p->ev[4], p->ev[7]) == hipSuccess) ms_rt += t;
+    }
+  }
+  float t = 0;
+  p->phase_ms[0] = (hipEventElapsedTime(&t, p->ev[0], p->ev[7]) == hipSuccess) ? t : -1;
+  p->phase_ms[1] = (hipEventElapsedTime(&t, p->ev[0], p->ev[1]) == hipSuccess) ? t : -1;
+  p->phase_ms[2] = ms_modes;
+  p->phase_ms[3] = ms_cascade;
+  p->phase_ms[4] = ms_rt;
+  p->phase_ms[5] = ms_eig;
+  return ORCWA_OK;
+}
+
+int orcwa_plan_sync(orcwa_plan* p) {
+  if (!p) return fail("null plan");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  HIPCHK(hipGetLastError());
+  return ORCWA_OK;
+}
+
+double orcwa_plan_phase_ms(orcwa_plan* p, int phase) {
+  if (!p || phase < 0 || phase > 5) return -1.0;
+  return p->phase_ms[phase];
+}
+
+int orcwa_plan_results(orcwa_plan* p, orcwa_result* out) {
+  if (!p || !out) return fail("null argument");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  HIPCHK(hipStreamSynchronize(s));
+  const int h = p->h, mp = p->maxpol;
+  std::vector<double> ref((size_t)p->count * mp * h), trn((size_t)p->count * mp * h);
+  std::vector<cd> amp((size_t)p->count * 8 * h);
+  std::vector<int> info(p->count), pinfo(std::max<long long>(1, p->nprob_total));
+  HIPCHK(hipMemcpy(ref.data(), p->d_ref, ref.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(trn.data(), p->d_trn, trn.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(amp.data(), p->d_amp, amp.size() * sizeof(cd), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(info.data(), p->d_info, info.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(pinfo.data(), p->d_pinfo, pinfo.size() * sizeof(int), hipMemcpyDeviceToHost));
+  int ginfo_any = 0;
+  if (p->G > 0) {
+    std::vector<int> gi(p->G);
+    HIPCHK(hipMemcpy(gi.data(), p->d_ginfo, gi.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int v : gi) ginfo_any |= v;
+  }
+  for (long long q = 0; q < p->nprob_total; ++q) info[p->prob_inst[q]] |= pinfo[q];
+  for (int i = 0; i < p->count; ++i) {
+    orcwa_result& r = out[i];
+    r.status = info[i] | ginfo_any;
+    const int np = p->npol[i];
+    if (r.ref)
+      std::memcpy(r.ref, ref.data() + (size_t)i * mp * h, (size_t)np * h * sizeof(double));
+    if (r.trn)
+      std::memcpy(r.trn, trn.data() + (size_t)i * mp * h, (size_t)np * h * sizeof(double));
+    if (r.amp) std::memcpy(r.amp, amp.data() + (size_t)i * 8 * h, (size_t)8 * h * sizeof(cd));
+  }
+  return ORCWA_OK;
+}
+
+int orcwa_gpu_solve_batch(orcwa_gpu_context* ctx, const orcwa_stack* stack,
+                          const orcwa_instance* inst, int count, orcwa_result* out) {
+  if (!out) return fail("null result array");
+  orcwa_plan* p = nullptr;
+  int rc = orcwa_plan_create(ctx, stack, inst, count, &p);
+  if (rc != ORCWA_OK) return rc;
+  rc = orcwa_plan_execute(p);
+  if (rc == ORCWA_OK) rc = orcwa_plan_sync(p);
+  if (rc == ORCWA_OK) rc = orcwa_plan_results(p, out);
+  orcwa_plan_destroy(p);
+  return rc;
+}
+
+}  // extern "C"

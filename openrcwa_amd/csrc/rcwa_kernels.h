@@ -1,0 +1,86 @@
+// Parameter records and launchers of the RCWA-specific kernels.
+#pragma once
+
+#include "engine.h"
+
+namespace orcwa {
+
+// geometry shared by a batch (device pointers for the edges)
+struct ConvGeom {
+  int nx, ny;          // harmonics per axis, 2N+1
+  int ncx, ncy;        // number of edges
+  int ncell;           // (ncx-1)*(ncy-1)
+  double Lx, Ly;       // periods
+  const double* cx;    // device [ncx]
+  const double* cy;    // device [ncy]
+};
+
+// one strip of one grid for the Li-rule series
+struct StripTask {
+  int grid;            // unique permittivity grid
+  int type;            // 0: y-strip (x series), 1: x-strip (y series)
+  int index;           // strip index
+  int pad;
+  long long out_off;   // element offset of the output record
+};
+
+// per-grid convolution products (device pointers, h x h column-major)
+struct GridMats {
+  const cd* E11;
+  const cd* E22;
+  const cd* S1;        // M E11        (M = fe33^-1)
+  const cd* S2;        // M Dm E11
+  const cd* S3;        // M E22
+  const cd* S4;        // M Dn E22
+};
+
+// one patterned-layer problem (instance, layer)
+struct ProbParam {
+  double kx0, ky0;     // Bloch offsets (1/length)
+  double inv_k0;       // 1/k0 (length)
+  int grid;            // grid index
+  int pad;
+};
+
+// one homogeneous-layer problem
+struct UniParam {
+  cd eps;
+  double kx0, ky0, inv_k0;
+  int inst;
+  int pad;
+};
+
+// plane-wave response of one instance
+struct RtParam {
+  cd eps_ref, eps_trn;
+  double kx0, ky0, inv_k0;
+  int npol;
+  int pad;
+  cd pol[8][2];        // (px, py)
+};
+
+// host launchers (all asynchronous on stream s)
+void launch_conv_strip(hipStream_t s, const ConvGeom& g, const StripTask* tasks, int ntasks,
+                       const cd* grids, cd* strips, size_t lds_bytes);
+void launch_conv_expand(hipStream_t s, const ConvGeom& g, const cd* strips,
+                        const long long* strip_base, cd* const* E11, cd* const* E22,
+                        cd* const* E33, int ngrid);
+void launch_scale_rows_order(hipStream_t s, const ConvGeom& g, int axis, const cd* const* in,
+                             cd* const* out, int ngrid);
+void launch_build_pq(hipStream_t s, const ConvGeom& g, const ProbParam* params, const GridMats* gm,
+                     cd* const* PQ, cd* const* PQcopy, int nprob);
+void launch_build_v(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* W,
+                    const cd* const* QW1, const cd* lam, cd* keff, cd* const* V, int* info,
+                    const int* inst_of, int nprob);
+void launch_zero_mats(hipStream_t s, long long count, cd* const* A, int batch);
+void launch_uniform(hipStream_t s, const ConvGeom& g, const UniParam* params, cd* keff,
+                    cd* const* V, cd* binv, int* info, int nuni);
+void launch_bdinv_apply(hipStream_t s, int h, const cd* const* binv, const cd* const* Vb,
+                        cd* const* H, int batch);
+void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const double* k0d,
+                        cd* const* R, cd* const* T, int batch);
+void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* const* R,
+               const cd* const* T, double* ref, double* trn, cd* amp, int* info, int max_pol,
+               int batch);
+
+}  // namespace orcwa

@@ -1,0 +1,539 @@
+// RCWA-specific kernels of the layer-stack solve on gfx950.
+//
+//   k_conv_strip / k_conv_expand : Li-rule convolution matrices
+//       fe11 (DiscontinuousX), fe22 (DiscontinuousY), fe33 (ContinuousXY)
+//       rcwa/FourierSolver2D.cpp:12-351, rcwa/FourierSolver1D.cpp:11-41,
+//       rcwa/ToeplitzMatrixXcs.cpp:8-44, rcwa/BlockToeplitzMatrixXcs.cpp:5-65
+//   k_build_pq   : P*Q of rcwa/Layer.cpp:150-220 + rcwa/WaveEquationSolver.cpp:30
+//   k_build_v    : V = Q W diag(1/keff), rcwa/WaveEquationSolver.cpp:33-36
+//   k_uniform    : closed-form modes of a homogeneous layer
+//   k_phase_scale, k_bdinv_apply : pieces of the RTCM cascade and interfaces
+//       rcwa/RCWASolver.cpp:378-614
+//   k_rt         : plane-wave response and per-order efficiencies
+//       rcwa/RCWASolver.cpp:1778-1908
+//
+// Units: every wave-vector is divided by k0 = 2 pi / lambda, so P*Q is
+// dimensionless; eigenvectors, V, interface matrices and R/T are unchanged
+// by that scaling, and the phase uses keff * (k0 d).
+#include "rcwa_kernels.h"
+
+namespace orcwa {
+
+static constexpr double kPi = 3.1415926535897932;  // rcwa/defns.h:25
+
+// exact Fourier coefficient F_m of f on cells [x_{j-1}, x_j] with period d
+// (rcwa/FourierSolver1D.cpp:11-41; sinc of rcwa/MathFunction.h:6-9)
+__device__ inline cd fourier_coeff(int m, const double* x, const cd* f, int ncell, double d,
+                                   int fstride) {
+  cd acc = mk(0.0, 0.0);
+  for (int j = 1; j <= ncell; ++j) {
+    const double x1 = x[j], x0 = x[j - 1];
+    const cd fj = f[(j - 1) * fstride];
+    if (m == 0) {
+      acc += ((x1 - x0) / d) * fj;
+    } else {
+      const double arg = m * (x1 - x0) / d;
+      const double sc = (arg == 0.0) ? 1.0 : sin(kPi * arg) / (kPi * arg);
+      double sn, cs;
+      sincos(-(double)m * kPi * (x1 + x0) / d, &sn, &cs);
+      acc += (sc * (x1 - x0) / d) * (fj * mk(cs, sn));
+    }
+  }
+  return acc;
+}
+
+// One workgroup per strip:
+//   type 0 (y-strip i, x-direction series): tX = inv(Toeplitz(F_x[1/eps_row])),
+//            fy1 = F_y[indicator of strip], ffi = F_x[eps_row]
+//   type 1 (x-strip i, y-direction series): tY = inv(Toeplitz(F_y[1/eps_col])),
+//            fx1 = F_x[indicator of strip]
+__global__ __launch_bounds__(256) void k_conv_strip(const ConvGeom g, const StripTask* tasks,
+                                                    const cd* grids, cd* strips) {
+  extern __shared__ cd sm[];
+  const StripTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int nxh = g.nx, nyh = g.ny;
+  const int ncx = g.ncx, ncy = g.ncy;
+  const cd* eps = grids + (long long)t.grid * g.ncell;
+  cd* out = strips + t.out_off;
+  const int mdim = (t.type == 0) ? nxh : nyh;
+  const int M = 2 * mdim - 1;
+  cd* sF = sm;                 // [M] coefficients of 1/eps
+  cd* sT = sm + M;             // [mdim*mdim] Toeplitz -> inverse
+  cd* sf = sT + mdim * mdim;   // [max(ncx,ncy)] 1/eps values
+  __shared__ int s_piv;
+  __shared__ int perm[64];
+  // 1/eps along the strip
+  if (t.type == 0) {
+    for (int j = tid; j < ncx - 1; j += 256) sf[j] = cinv(eps[t.index * (ncx - 1) + j]);
+  } else {
+    for (int j = tid; j < ncy - 1; j += 256) sf[j] = cinv(eps[j * (ncx - 1) + t.index]);
+  }
+  __syncthreads();
+  for (int i = tid; i < M; i += 256) {
+    const int m = i - (M - 1) / 2;
+    sF[i] = (t.type == 0) ? fourier_coeff(m, g.cx, sf, ncx - 1, g.Lx, 1)
+                          : fourier_coeff(m, g.cy, sf, ncy - 1, g.Ly, 1);
+  }
+  __syncthreads();
+  // T[r,c] = F[(r-c) + mdim-1], column-major in LDS
+  for (int e = tid; e < mdim * mdim; e += 256) {
+    const int r = e % mdim, c = e / mdim;
+    sT[e] = sF[(r - c) + mdim - 1];
+  }
+  __syncthreads();
+  // in-place Gauss-Jordan inverse with partial pivoting (rows), columns
+  // unscrambled at the end
+  for (int k = 0; k < mdim; ++k) {
+    if (tid == 0) {
+      int p = k;
+      double best = cabs(sT[k + k * mdim]);
+      for (int i = k + 1; i < mdim; ++i) {
+        double v = cabs(sT[i + k * mdim]);
+        if (v > best) { best = v; p = i; }
+      }
+      s_piv = p;
+      perm[k] = p;
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != k)
+      for (int c = tid; c < mdim; c += 256) {
+        cd a = sT[k + c * mdim];
+        sT[k + c * mdim] = sT[p + c * mdim];
+        sT[p + c * mdim] = a;
+      }
+    __syncthreads();
+    const cd piv = sT[k + k * mdim];
+    const cd ip = cinv(piv);
+    __syncthreads();
+    for (int c = tid; c < mdim; c += 256) {
+      if (c == k) sT[k + c * mdim] = ip;
+      else sT[k + c * mdim] = sT[k + c * mdim] * ip;
+    }
+    __syncthreads();
+    for (int e = tid; e < mdim * mdim; e += 256) {
+      const int i = e % mdim, c = e / mdim;
+      if (i == k) continue;
+      const cd f = sT[i + k * mdim];
+      if (c == k) continue;
+      sT[e] -= f * sT[k + c * mdim];
+    }
+    __syncthreads();
+    for (int i = tid; i < mdim; i += 256)
+      if (i != k) sT[i + k * mdim] = -(sT[i + k * mdim] * ip);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int k = mdim - 1; k >= 0; --k) {
+      const int p = perm[k];
+      if (p != k)
+        for (int r = 0; r < mdim; ++r) {
+          cd a = sT[r + k * mdim];
+          sT[r + k * mdim] = sT[r + p * mdim];
+          sT[r + p * mdim] = a;
+        }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < mdim * mdim; e += 256) out[e] = sT[e];
+  // indicator series of the strip along the other axis, and (type 0) the
+  // series of eps along x
+  cd one = mk(1.0, 0.0);
+  if (t.type == 0) {
+    const int My = 2 * nyh - 1, Mx = 2 * nxh - 1;
+    double yy[2] = {g.cy[t.index], g.cy[t.index + 1]};
+    for (int i = tid; i < My; i += 256)
+      out[mdim * mdim + i] = fourier_coeff(i - (My - 1) / 2, yy, &one, 1, g.Ly, 0);
+    for (int i = tid; i < Mx; i += 256)
+      out[mdim * mdim + My + i] =
+          fourier_coeff(i - (Mx - 1) / 2, g.cx, eps + t.index * (ncx - 1), ncx - 1, g.Lx, 1);
+  } else {
+    const int Mx = 2 * nxh - 1;
+    double xx[2] = {g.cx[t.index], g.cx[t.index + 1]};
+    for (int i = tid; i < Mx; i += 256)
+      out[mdim * mdim + i] = fourier_coeff(i - (Mx - 1) / 2, xx, &one, 1, g.Lx, 0);
+  }
+}
+
+// fe11, fe22, fe33 (h x h, column-major) of every grid.  Row r = ii*ny+jj,
+// column c = kk*ny+ll.
+__global__ void k_conv_expand(const ConvGeom g, const cd* strips, const long long* strip_base,
+                              cd* const* E11, cd* const* E22, cd* const* E33) {
+  const int gi = blockIdx.y;
+  const int h = g.nx * g.ny;
+  const long long hh = (long long)h * h;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= hh) return;
+  const int r = (int)(e % h), c = (int)(e / h);
+  const int ii = r / g.ny, jj = r % g.ny, kk = c / g.ny, ll = c % g.ny;
+  const int nx = g.nx, ny = g.ny;
+  const int My = 2 * ny - 1, Mx = 2 * nx - 1;
+  const long long sx = (long long)nx * nx + My + Mx;  // stride of a type-0 strip record
+  const long long sy = (long long)ny * ny + Mx;       // stride of a type-1 strip record
+  const cd* base0 = strips + strip_base[2 * gi];
+  const cd* base1 = strips + strip_base[2 * gi + 1];
+  cd f11 = mk(0.0, 0.0), f22 = mk(0.0, 0.0), f33 = mk(0.0, 0.0);
+  for (int i = 0; i < g.ncy - 1; ++i) {
+    const cd* rec = base0 + i * sx;
+    const cd fy = rec[nx * nx + (jj - ll) + ny - 1];
+    f11 += fy * rec[ii + kk * nx];
+    f33 += rec[nx * nx + My + (ii - kk) + nx - 1] * fy;
+  }
+  for (int i = 0; i < g.ncx - 1; ++i) {
+    const cd* rec = base1 + i * sy;
+    f22 += rec[ny * ny + (ii - kk) + nx - 1] * rec[jj + ll * ny];
+  }
+  E11[gi][e] = f11;
+  E22[gi][e] = f22;
+  E33[gi][e] = f33;
+}
+
+// out[r, c] = d_r * in[r, c] with d_r = 2 pi (order index) / period along
+// the chosen axis (Dm for axis 0, Dn for axis 1)
+__global__ void k_scale_rows_order(const ConvGeom g, int axis, const cd* const* in,
+                                   cd* const* out) {
+  const int h = g.nx * g.ny;
+  const long long hh = (long long)h * h;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= hh) return;
+  const int r = (int)(e % h);
+  const double d = (axis == 0) ? 2.0 * kPi * (r / g.ny - (g.nx - 1) / 2) / g.Lx
+                               : 2.0 * kPi * (r % g.ny - (g.ny - 1) / 2) / g.Ly;
+  out[blockIdx.y][e] = d * in[blockIdx.y][e];
+}
+
+__device__ inline void k_vectors(const ConvGeom& g, const ProbParam& p, int r, double& kx,
+                                 double& ky) {
+  // alpha_k = 2 pi m / Lx + kx0, beta_k = 2 pi n / Ly + ky0 (rcwa/RCWASolver.cpp:242-266),
+  // divided by k0
+  const int m = r / g.ny - (g.nx - 1) / 2;
+  const int n = r % g.ny - (g.ny - 1) / 2;
+  kx = (2.0 * kPi * m / g.Lx + p.kx0) * p.inv_k0;
+  ky = (2.0 * kPi * n / g.Ly + p.ky0) * p.inv_k0;
+}
+
+// P*Q for a patterned non-magnetic layer, in k0 units, from the per-grid
+// products S1 = M E11, S2 = M Dm E11, S3 = M E22, S4 = M Dn E22 (M = fe33^-1):
+//   A1 = M Kx E11 = (kx0/k0) S1 + S2/k0,  A2 = M Ky E22 = (ky0/k0) S3 + S4/k0
+//   PQ = [[E11 - Ky^2 - Kx A1, KxKy - Kx A2], [KxKy - Ky A1, E22 - Kx^2 - Ky A2]]
+// which equals P*Q of rcwa/Layer.cpp:150-220 after cancelling the diagonal
+// terms (Kx, Ky diagonal).
+__global__ void k_build_pq(const ConvGeom g, const ProbParam* params, const GridMats* gm,
+                           cd* const* PQ, cd* const* PQcopy) {
+  const int pi = blockIdx.y;
+  const ProbParam p = params[pi];
+  const GridMats G = gm[p.grid];
+  const int h = g.nx * g.ny, n = 2 * h;
+  const long long hh = (long long)h * h;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= hh) return;
+  const int r = (int)(e % h), c = (int)(e / h);
+  double kxr, kyr;
+  k_vectors(g, p, r, kxr, kyr);
+  const double ax = p.kx0 * p.inv_k0, ay = p.ky0 * p.inv_k0;
+  const cd a1 = ax * G.S1[e] + p.inv_k0 * G.S2[e];
+  const cd a2 = ay * G.S3[e] + p.inv_k0 * G.S4[e];
+  cd v11 = G.E11[e] - kxr * a1;
+  cd v21 = -(kyr * a1);
+  cd v12 = -(kxr * a2);
+  cd v22 = G.E22[e] - kyr * a2;
+  if (r == c) {
+    v11.re -= kyr * kyr;
+    v21.re += kxr * kyr;
+    v12.re += kxr * kyr;
+    v22.re -= kxr * kxr;
+  }
+  cd* P = PQ[pi];
+  cd* P2 = PQcopy[pi];
+  const long long i11 = r + (long long)c * n, i21 = (h + r) + (long long)c * n;
+  const long long i12 = r + (long long)(h + c) * n, i22 = (h + r) + (long long)(h + c) * n;
+  P[i11] = v11; P[i21] = v21; P[i12] = v12; P[i22] = v22;
+  P2[i11] = v11; P2[i21] = v21; P2[i12] = v12; P2[i22] = v22;
+}
+
+// keff = GammaSqrtOp(lambda) (rcwa/WaveEquationSolver.cpp:32), then
+// V[:, c] = (Q W)[:, c] / keff_c with
+//   Q = [[-KxKy, Kx^2 - E22], [E11 - Ky^2, KyKx]]  (rcwa/Layer.cpp:214-220, k0 = 1)
+// QW1 holds E22 W_bottom (rows 0..h-1) and E11 W_top (rows h..n-1).
+__global__ void k_build_v(const ConvGeom g, const ProbParam* params, const cd* const* W,
+                          const cd* const* QW1, const cd* lam, cd* keff_all, cd* const* V,
+                          int* info, const int* inst_of) {
+  const int pi = blockIdx.y;
+  const ProbParam p = params[pi];
+  const int h = g.nx * g.ny, n = 2 * h;
+  const long long nn = (long long)n * n;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  const cd kf = gamma_sqrt(lam[(long long)pi * n + c]);
+  if (r == 0) {
+    keff_all[(long long)pi * n + c] = kf;
+    if (kf.re == 0.0 && kf.im == 0.0) atomicOr(&info[inst_of[pi]], ORCWA_STATUS_SINGULAR);
+  }
+  const int rr = (r < h) ? r : r - h;
+  double kx, ky;
+  k_vectors(g, p, rr, kx, ky);
+  const cd* w = W[pi] + (long long)c * n;
+  const cd* q = QW1[pi] + (long long)c * n;
+  cd v;
+  if (r < h) v = (-(kx * ky)) * w[rr] + (kx * kx) * w[h + rr] - q[rr];
+  else v = q[h + rr] - (ky * ky) * w[rr] + (ky * kx) * w[h + rr];
+  V[pi][e] = (kf.re == 0.0 && kf.im == 0.0) ? v : cdiv(v, kf);
+}
+
+__global__ void k_zero_mats(long long count, cd* const* A) {
+  cd* a = A[blockIdx.y];
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x)
+    a[e] = mk(0.0, 0.0);
+}
+
+// Homogeneous layer (eps, mu = 1): W = I, keff = GammaSqrtOp(eps - kx^2 - ky^2)
+// for both field components of every order, V = Q diag(1/keff) which is
+// 2x2-block diagonal per order.  Writes keff (n), the non-zero entries of the
+// dense V (zeroed beforehand by k_zero_mats) and the inverse 2x2 blocks
+// (4h: b00, b01, b10, b11 of V^{-1}).
+__global__ void k_uniform(const ConvGeom g, const UniParam* params, cd* keff_all,
+                          cd* const* Vdense, cd* binv_all, int* info) {
+  const int ui = blockIdx.y;
+  const UniParam u = params[ui];
+  const int h = g.nx * g.ny, n = 2 * h;
+  const long long nn = (long long)n * n;
+  cd* V = Vdense[ui];
+  (void)nn;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= h) return;
+  ProbParam p;
+  p.kx0 = u.kx0; p.ky0 = u.ky0; p.inv_k0 = u.inv_k0; p.grid = 0;
+  double kx, ky;
+  k_vectors(g, p, k, kx, ky);
+  const cd kz2 = u.eps - mk(kx * kx + ky * ky, 0.0);
+  const cd kf = gamma_sqrt(kz2);
+  keff_all[(long long)ui * n + k] = kf;
+  keff_all[(long long)ui * n + h + k] = kf;
+  if (kf.re == 0.0 && kf.im == 0.0) {
+    atomicOr(&info[u.inst], ORCWA_STATUS_SINGULAR);
+    return;
+  }
+  const cd ik = cinv(kf);
+  const cd v00 = (-(kx * ky)) * ik;
+  const cd v01 = (mk(kx * kx, 0.0) - u.eps) * ik;
+  const cd v10 = (u.eps - mk(ky * ky, 0.0)) * ik;
+  const cd v11 = (ky * kx) * ik;
+  V[k + (long long)k * n] = v00;
+  V[k + (long long)(h + k) * n] = v01;
+  V[(h + k) + (long long)k * n] = v10;
+  V[(h + k) + (long long)(h + k) * n] = v11;
+  const cd det = v00 * v11 - v01 * v10;
+  cd* bi = binv_all + (long long)ui * 4 * h;
+  if (det.re == 0.0 && det.im == 0.0) {
+    atomicOr(&info[u.inst], ORCWA_STATUS_SINGULAR);
+    bi[k] = bi[h + k] = bi[2 * h + k] = bi[3 * h + k] = mk(0.0, 0.0);
+    return;
+  }
+  const cd idet = cinv(det);
+  bi[k] = v11 * idet;
+  bi[h + k] = -(v01 * idet);
+  bi[2 * h + k] = -(v10 * idet);
+  bi[3 * h + k] = v00 * idet;
+}
+
+// H = V_a^{-1} V_b with V_a 2x2-block diagonal (inverse blocks binv)
+__global__ void k_bdinv_apply(int h, const cd* const* binv, const cd* const* Vb, cd* const* Hout) {
+  const int b = blockIdx.y;
+  const int n = 2 * h;
+  const long long cnt = (long long)h * n;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= cnt) return;
+  const int k = (int)(e % h), c = (int)(e / h);
+  const cd* bi = binv[b];
+  const cd* v = Vb[b] + (long long)c * n;
+  const cd x0 = v[k], x1 = v[h + k];
+  cd* o = Hout[b] + (long long)c * n;
+  o[k] = bi[k] * x0 + bi[h + k] * x1;
+  o[h + k] = bi[2 * h + k] * x0 + bi[3 * h + k] * x1;
+}
+
+// R <- Phi R Phi, T <- T Phi with phi = exp(i keff (k0 d))
+// (rcwa/RCWASolver.cpp:415-434)
+__global__ void k_phase_scale(int n, const cd* const* keff, const double* k0d, cd* const* R,
+                              cd* const* T) {
+  const int b = blockIdx.y;
+  const long long nn = (long long)n * n;
+  const double kd = k0d[b];
+  const cd* kf = keff[b];
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nn;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    const cd ar = kd * kf[r], ac = kd * kf[c];
+    const cd pr = cexp(mk(-ar.im, ar.re));
+    const cd pc = cexp(mk(-ac.im, ac.re));
+    R[b][e] = pr * R[b][e] * pc;
+    T[b][e] = T[b][e] * pc;
+  }
+}
+
+// Plane-wave response and efficiencies (rcwa/RCWASolver.cpp:1778-1908) for a
+// stack whose first and last layers are homogeneous (W = I there): the
+// incident coefficient vector is s = [px delta; py delta], r = Rud s,
+// t = Tdd s.  One workgroup per instance.
+__global__ __launch_bounds__(256) void k_rt(const ConvGeom g, const RtParam* params,
+                                            const cd* const* Rud, const cd* const* Tdd,
+                                            double* ref_out, double* trn_out, cd* amp_out,
+                                            int* info, int max_pol) {
+  const int b = blockIdx.x;
+  const RtParam p = params[b];
+  const int h = g.nx * g.ny, n = 2 * h;
+  const int i0 = g.ny * (g.nx - 1) / 2 + (g.ny - 1) / 2;
+  const cd* R = Rud[b];
+  const cd* T = Tdd[b];
+  cd* amp = amp_out + (long long)b * 8 * h;
+  __shared__ int s_graze;
+  if (threadIdx.x == 0) s_graze = 0;
+  __syncthreads();
+  ProbParam pp;
+  pp.kx0 = p.kx0; pp.ky0 = p.ky0; pp.inv_k0 = p.inv_k0; pp.grid = 0;
+  double kzinc = 0.0;
+  {
+    double kx, ky;
+    k_vectors(g, pp, i0, kx, ky);
+    const cd kzi = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
+    kzinc = fabs(kzi.re);
+  }
+  for (int k = threadIdx.x; k < h; k += blockDim.x) {
+    // basis responses: column i0 (x-polarised) and h+i0 (y-polarised)
+    amp[0 * h + k] = R[k + (long long)i0 * n];
+    amp[1 * h + k] = R[h + k + (long long)i0 * n];
+    amp[2 * h + k] = T[k + (long long)i0 * n];
+    amp[3 * h + k] = T[h + k + (long long)i0 * n];
+    amp[4 * h + k] = R[k + (long long)(h + i0) * n];
+    amp[5 * h + k] = R[h + k + (long long)(h + i0) * n];
+    amp[6 * h + k] = T[k + (long long)(h + i0) * n];
+    amp[7 * h + k] = T[h + k + (long long)(h + i0) * n];
+    double kx, ky;
+    k_vectors(g, pp, k, kx, ky);
+    const cd kzr = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
+    const cd kzt = csqrt_principal(p.eps_trn - mk(kx * kx + ky * ky, 0.0));
+    if ((kzr.re == 0.0 && kzr.im == 0.0) || (kzt.re == 0.0 && kzt.im == 0.0)) s_graze = 1;
+  }
+  __syncthreads();
+  const bool graze = s_graze || (kzinc < 1e-12);
+  if (threadIdx.x == 0 && graze) info[b] |= ORCWA_STATUS_GRAZING;
+  for (int pol = 0; pol < max_pol; ++pol) {
+    double* ro = ref_out + ((long long)b * max_pol + pol) * h;
+    double* to = trn_out + ((long long)b * max_pol + pol) * h;
+    if (pol >= p.npol) continue;
+    const cd px = p.pol[pol][0], py = p.pol[pol][1];
+    for (int k = threadIdx.x; k < h; k += blockDim.x) {
+      if (graze) {
+        ro[k] = 0.0;
+        to[k] = 0.0;
+        continue;
+      }
+      const cd rx = px * R[k + (long long)i0 * n] + py * R[k + (long long)(h + i0) * n];
+      const cd ry = px * R[h + k + (long long)i0 * n] + py * R[h + k + (long long)(h + i0) * n];
+      const cd tx = px * T[k + (long long)i0 * n] + py * T[k + (long long)(h + i0) * n];
+      const cd ty = px * T[h + k + (long long)i0 * n] + py * T[h + k + (long long)(h + i0) * n];
+      double kx, ky;
+      k_vectors(g, pp, k, kx, ky);
+      const cd kzr = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
+      const cd kzt = csqrt_principal(p.eps_trn - mk(kx * kx + ky * ky, 0.0));
+      const cd rz = cdiv(kx * rx + ky * ry, kzr);
+      const cd tz = cdiv(kx * tx + ky * ty, kzt);
+      const double rr = norm2(rx) + norm2(ry) + norm2(rz);
+      const double tt = norm2(tx) + norm2(ty) + norm2(tz);
+      ro[k] = fabs(kzr.re) * rr / kzinc;
+      to[k] = fabs(kzt.re) * tt / kzinc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(long long count, int per = 256) {
+  long long b = (count + per - 1) / per;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+void launch_conv_strip(hipStream_t s, const ConvGeom& g, const StripTask* tasks, int ntasks,
+                       const cd* grids, cd* strips, size_t lds_bytes) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_conv_strip, dim3(ntasks), dim3(256), lds_bytes, s, g, tasks, grids, strips);
+}
+
+void launch_conv_expand(hipStream_t s, const ConvGeom& g, const cd* strips,
+                        const long long* strip_base, cd* const* E11, cd* const* E22,
+                        cd* const* E33, int ngrid) {
+  if (ngrid <= 0) return;
+  const long long hh = (long long)g.nx * g.ny * g.nx * g.ny;
+  hipLaunchKernelGGL(k_conv_expand, dim3(blocks_for(hh), ngrid), dim3(256), 0, s, g, strips,
+                     strip_base, E11, E22, E33);
+}
+
+void launch_scale_rows_order(hipStream_t s, const ConvGeom& g, int axis, const cd* const* in,
+                             cd* const* out, int ngrid) {
+  if (ngrid <= 0) return;
+  const long long hh = (long long)g.nx * g.ny * g.nx * g.ny;
+  hipLaunchKernelGGL(k_scale_rows_order, dim3(blocks_for(hh), ngrid), dim3(256), 0, s, g, axis,
+                     in, out);
+}
+
+void launch_build_pq(hipStream_t s, const ConvGeom& g, const ProbParam* params, const GridMats* gm,
+                     cd* const* PQ, cd* const* PQcopy, int nprob) {
+  if (nprob <= 0) return;
+  const long long hh = (long long)g.nx * g.ny * g.nx * g.ny;
+  hipLaunchKernelGGL(k_build_pq, dim3(blocks_for(hh), nprob), dim3(256), 0, s, g, params, gm, PQ,
+                     PQcopy);
+}
+
+void launch_build_v(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* W,
+                    const cd* const* QW1, const cd* lam, cd* keff, cd* const* V, int* info,
+                    const int* inst_of, int nprob) {
+  if (nprob <= 0) return;
+  const long long n = 2LL * g.nx * g.ny;
+  hipLaunchKernelGGL(k_build_v, dim3(blocks_for(n * n), nprob), dim3(256), 0, s, g, params, W,
+                     QW1, lam, keff, V, info, inst_of);
+}
+
+void launch_zero_mats(hipStream_t s, long long count, cd* const* A, int batch) {
+  if (batch <= 0) return;
+  long long b = blocks_for(count);
+  hipLaunchKernelGGL(k_zero_mats, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s,
+                     count, A);
+}
+
+void launch_uniform(hipStream_t s, const ConvGeom& g, const UniParam* params, cd* keff,
+                    cd* const* V, cd* binv, int* info, int nuni) {
+  if (nuni <= 0) return;
+  const long long h = (long long)g.nx * g.ny;
+  launch_zero_mats(s, 4 * h * h, V, nuni);
+  hipLaunchKernelGGL(k_uniform, dim3(blocks_for(h), nuni), dim3(256), 0, s, g, params, keff, V,
+                     binv, info);
+}
+
+void launch_bdinv_apply(hipStream_t s, int h, const cd* const* binv, const cd* const* Vb,
+                        cd* const* H, int batch) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_bdinv_apply, dim3(blocks_for(2LL * h * h), batch), dim3(256), 0, s, h,
+                     binv, Vb, H);
+}
+
+void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const double* k0d,
+                        cd* const* R, cd* const* T, int batch) {
+  if (batch <= 0) return;
+  long long b = blocks_for((long long)n * n);
+  hipLaunchKernelGGL(k_phase_scale, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s,
+                     n, keff, k0d, R, T);
+}
+
+void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* const* R,
+               const cd* const* T, double* ref, double* trn, cd* amp, int* info, int max_pol,
+               int batch) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_rt, dim3(batch), dim3(256), 0, s, g, params, R, T, ref, trn, amp, info,
+                     max_pol);
+}
+
+}  // namespace orcwa

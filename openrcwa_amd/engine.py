@@ -1,0 +1,174 @@
+"""Host-side API over the C ABI: contexts, plans and batched solves.
+
+A batch is many independent instances of one stack layout (same harmonics,
+cell edges and layer count): wavelength, Bloch vector, per-layer permittivity
+grid and incident polarisations vary per instance.  Lengths and wavelengths
+share one unit, as in the reference (metres on the .ofd path,
+sol/rcwa_bridge.cpp:15-16; micrometres on the .orcwa path).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+class Context:
+    """One device + one stream (orcwa_gpu_create)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        h = c_void_p()
+        _lib.check(self._lib.orcwa_gpu_create(ctypes.byref(h), int(device)), "orcwa_gpu_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.orcwa_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+@dataclass
+class BatchResult:
+    ref: np.ndarray      # (B, npol, h) per-order reflection efficiencies
+    trn: np.ndarray      # (B, npol, h)
+    amp: np.ndarray      # (B, 2, 4, h) complex basis responses (rx, ry, tx, ty)
+    status: np.ndarray   # (B,) orcwa_status bits
+
+    @property
+    def R(self):
+        return self.ref.sum(axis=-1)
+
+    @property
+    def T(self):
+        return self.trn.sum(axis=-1)
+
+
+class Plan:
+    """orcwa_plan_*: inputs copied to device memory once; execute() runs the
+    whole batch on the GPU."""
+
+    def __init__(self, ctx: Context, nX: int, nY: int, coord_x, coord_y, thickness,
+                 lam, kx0, ky0, eps, pols):
+        self._lib = _lib.load()
+        self.ctx = ctx
+        self.coord_x = np.ascontiguousarray(coord_x, dtype=np.float64)
+        self.coord_y = np.ascontiguousarray(coord_y, dtype=np.float64)
+        self.thickness = np.ascontiguousarray(thickness, dtype=np.float64)
+        L = self.thickness.size
+        ncx, ncy = self.coord_x.size, self.coord_y.size
+        lam = np.atleast_1d(np.asarray(lam, dtype=np.float64))
+        B = lam.size
+        kx0 = np.broadcast_to(np.asarray(kx0, dtype=np.float64), (B,))
+        ky0 = np.broadcast_to(np.asarray(ky0, dtype=np.float64), (B,))
+        eps = np.asarray(eps, dtype=np.complex128)
+        shared = eps.ndim == 3
+        if shared:
+            eps = eps.reshape(1, L, ncy - 1, ncx - 1)
+        if eps.shape[1:] != (L, ncy - 1, ncx - 1) or eps.shape[0] not in (1, B):
+            raise ValueError(f"eps must be (B|1, L, ncy-1, ncx-1); got {eps.shape}")
+        self.eps = np.ascontiguousarray(eps)
+        pols = np.asarray(pols, dtype=np.complex128)
+        if pols.ndim == 2:
+            pols = np.broadcast_to(pols, (B,) + pols.shape)
+        if pols.shape[0] != B or pols.shape[2] != 2 or not 1 <= pols.shape[1] <= 8:
+            raise ValueError("pols must be (B, npol<=8, 2) complex (px, py)")
+        self.npol = pols.shape[1]
+        self.pols = np.ascontiguousarray(pols)
+        self.B = B
+        self.nX, self.nY = int(nX), int(nY)
+        self.h = (2 * self.nX + 1) * (2 * self.nY + 1)
+
+        st = _lib.OrcwaStack()
+        st.nx_harm, st.ny_harm = self.nX, self.nY
+        st.n_cx, st.n_cy = ncx, ncy
+        st.coord_x, st.coord_y = _dptr(self.coord_x), _dptr(self.coord_y)
+        st.n_layers = L
+        st.thickness = _dptr(self.thickness)
+        self._stack = st
+        inst = (_lib.OrcwaInstance * B)()
+        eps_flat = self.eps.view(np.float64).reshape(self.eps.shape[0], -1)
+        pol_flat = self.pols.view(np.float64).reshape(B, -1)
+        self._eps_flat, self._pol_flat = eps_flat, pol_flat
+        for i in range(B):
+            inst[i].lam = float(lam[i])
+            inst[i].kx0 = float(kx0[i])
+            inst[i].ky0 = float(ky0[i])
+            inst[i].eps = _dptr(eps_flat[0 if shared or self.eps.shape[0] == 1 else i])
+            inst[i].n_pol = self.npol
+            inst[i].pol = _dptr(pol_flat[i])
+        self._inst = inst
+        h = c_void_p()
+        _lib.check(self._lib.orcwa_plan_create(ctx.handle, ctypes.byref(st), inst, B,
+                                               ctypes.byref(h)), "orcwa_plan_create")
+        self._h = h
+
+    def execute(self):
+        _lib.check(self._lib.orcwa_plan_execute(self._h), "orcwa_plan_execute")
+
+    def sync(self):
+        _lib.check(self._lib.orcwa_plan_sync(self._h), "orcwa_plan_sync")
+
+    def phase_ms(self, phase: int) -> float:
+        return float(self._lib.orcwa_plan_phase_ms(self._h, int(phase)))
+
+    def results(self) -> BatchResult:
+        B, npol, h = self.B, self.npol, self.h
+        ref = np.zeros((B, npol, h))
+        trn = np.zeros((B, npol, h))
+        amp = np.zeros((B, 2, 4, h), dtype=np.complex128)
+        res = (_lib.OrcwaResult * B)()
+        for i in range(B):
+            res[i].ref = _dptr(ref[i])
+            res[i].trn = _dptr(trn[i])
+            res[i].amp = amp[i].ctypes.data_as(POINTER(c_double))
+        _lib.check(self._lib.orcwa_plan_results(self._h, res), "orcwa_plan_results")
+        status = np.array([res[i].status for i in range(B)], dtype=np.int32)
+        return BatchResult(ref, trn, amp, status)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.orcwa_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_batch(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps,
+                pols) -> BatchResult:
+    """Plan + execute + results (orcwa_gpu_solve_batch semantics)."""
+    p = Plan(ctx, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps, pols)
+    try:
+        p.execute()
+        p.sync()
+        return p.results()
+    finally:
+        p.close()

@@ -1,0 +1,106 @@
+"""Seeded synthetic workloads of the layer-stack solve.
+
+``pillar2d`` is the 2D rectangular-pillar batch: Nx = Ny = 7 (15 x 15 = 225
+orders, 450 x 450 layer matrices), periods 400 x 400 nm, superstrate eps = 1,
+four patterned layers, substrate eps = 2.25.  Each patterned layer holds one
+eps1 rectangle [0, fx Lx] x [0, fy Ly] with fx, fy ~ U(0.2, 0.8),
+eps1 = U(2, 12) + i U(0, 0.1), eps2 = 1, thickness ~ U(50, 300) nm.  The batch
+is 256 frequencies (4.3e14 .. 7.5e14 Hz, linear) x 8 incidence directions
+(theta in {0, 35/3, 70/3, 35} deg x phi in {0, 45} deg), TE and TM.
+
+``grating1d`` is data/sample/grating.ofd with N = 5: period 300 nm, air /
+eps 4.0|1.0 at fill 0.5, 200 nm / eps 2.25, normal incidence, 16 frequencies
+4.3e14 .. 7.5e14 Hz.
+
+Random numbers: splitmix64, 53-bit uniforms in [0, 1).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .ofd import C0, common_edges, layer_grid, polarisation_basis
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+
+PILLAR_SEED = 20261202
+
+
+def pillar2d(n_freq=256, thetas=(0.0, 35.0 / 3.0, 70.0 / 3.0, 35.0), phis=(0.0, 45.0),
+             nX=7, nY=7, seed=PILLAR_SEED, n_layers=4, period=400e-9):
+    """The 2D pillar workload as batch arrays (instance order: frequency-major,
+    then theta, then phi).  Lengths in metres."""
+    rng = SplitMix64(seed)
+    lay = []
+    for _ in range(n_layers):
+        fx = rng.uniform(0.2, 0.8)
+        fy = rng.uniform(0.2, 0.8)
+        e1 = complex(rng.uniform(2.0, 12.0), rng.uniform(0.0, 0.1))
+        d = rng.uniform(50e-9, 300e-9)
+        lay.append(dict(eps1=e1.real, eps1i=e1.imag, eps2=1.0, eps2i=0.0, fill=fx, fill_y=fy,
+                        thickness=d))
+    layers = ([dict(eps1=1.0, eps2=1.0, fill=0.5, fill_y=0.5, thickness=0.0)] + lay +
+              [dict(eps1=2.25, eps2=2.25, fill=0.5, fill_y=0.5, thickness=0.0)])
+    L = len(layers)
+    xe = common_edges(period, [l["fill"] for l in layers])
+    ye = common_edges(period, [l["fill_y"] for l in layers])
+    eps = np.stack([layer_grid(l["fill"], l["fill_y"], complex(l["eps1"], l.get("eps1i", 0.0)),
+                               complex(l["eps2"], l.get("eps2i", 0.0)), xe, ye) for l in layers])
+    thick = np.array([0.0 if i in (0, L - 1) else layers[i]["thickness"] for i in range(L)])
+    freqs = np.linspace(4.3e14, 7.5e14, n_freq)
+    lam, kx0, ky0, pols, ang = [], [], [], [], []
+    for f in freqs:
+        lm = C0 / f
+        k0 = 2.0 * math.pi / lm
+        for th in thetas:
+            for ph in phis:
+                t, p = math.radians(th), math.radians(ph)
+                lam.append(lm)
+                kx0.append(k0 * math.sin(t) * math.cos(p))   # n_inc = 1
+                ky0.append(k0 * math.sin(t) * math.sin(p))
+                te, tm = polarisation_basis(th, ph)
+                pols.append([[te[0], te[1]], [tm[0], tm[1]]])
+                ang.append((th, ph))
+    lam = np.array(lam)
+    kx0 = np.array(kx0)
+    ky0 = np.array(ky0)
+    pols = np.array(pols, dtype=complex)
+    # Rayleigh anomaly flags: |kz| < 1e-9 k0 for some order in either half-space
+    k0v = 2.0 * math.pi / lam
+    m = np.repeat(np.arange(-nX, nX + 1), 2 * nY + 1)
+    n = np.tile(np.arange(-nY, nY + 1), 2 * nX + 1)
+    kx = kx0[:, None] + 2 * math.pi * m[None, :] / period
+    ky = ky0[:, None] + 2 * math.pi * n[None, :] / period
+    kt = kx * kx + ky * ky
+    rayleigh = np.zeros(lam.size, dtype=bool)
+    for e in (1.0, 2.25):
+        rayleigh |= (np.sqrt(np.abs(e * k0v[:, None] ** 2 - kt)) < 1e-9 * k0v[:, None]).any(1)
+    return dict(nX=nX, nY=nY, coord_x=np.array(xe), coord_y=np.array(ye), thickness=thick,
+                eps=eps, lam=lam, kx0=kx0, ky0=ky0, pols=pols, angles=ang, layers=layers,
+                period=period, rayleigh=rayleigh, freqs=freqs)
+
+
+def grating1d(nX=5, n_freq=16):
+    """data/sample/grating.ofd with rcwa = nX 3.0e-7 and n_freq frequencies."""
+    layers = [dict(eps1=1.0, eps2=1.0, fill=0.5, thickness=0.0),
+              dict(eps1=4.0, eps2=1.0, fill=0.5, thickness=2.0e-7),
+              dict(eps1=2.25, eps2=2.25, fill=0.5, thickness=0.0)]
+    freqs = np.linspace(4.3e14, 7.5e14, n_freq)
+    return dict(nX=nX, period=3.0e-7, layers=layers, freqs=freqs)

@@ -1,0 +1,138 @@
+"""GPU component checks, one kernel family at a time, against numpy (LAPACK)
+and the FP64 oracle.  Bar: FP64 round-off (relative errors ~1e-13)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import rcwa_oracle as o
+from tests import gpu_helpers as g
+
+pytestmark = pytest.mark.gpu
+rng = np.random.default_rng(7)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (100, 70, 130),
+                                   (450, 450, 450)])
+def test_zgemm_shapes(gpu_ctx, M, N, K):
+    A, B, C = crand(2, M, K), crand(2, K, N), crand(2, M, N)
+    al, be = 0.7 - 0.2j, -1.1 + 0.4j
+    out = g.zgemm(gpu_ctx, A, B, C, al, be)
+    ref = al * A @ B + be * C
+    scale = np.abs(A).max() * np.abs(B).max() * K + np.abs(C).max()
+    assert np.abs(out - ref).max() / scale < 1e-14
+
+
+@pytest.mark.parametrize("tA,cA,tB,cB", [(1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 1), (1, 1, 1, 0)])
+def test_zgemm_ops(gpu_ctx, tA, cA, tB, cB):
+    M, N, K = 37, 29, 41
+    A = crand(3, K, M) if tA else crand(3, M, K)
+    B = crand(3, N, K) if tB else crand(3, K, N)
+    C = np.zeros((3, M, N), complex)
+    out = g.zgemm(gpu_ctx, A, B, C, 1.0 + 0j, 0j, tA, cA, tB, cB)
+    opA = np.transpose(A, (0, 2, 1)) if tA else A
+    opB = np.transpose(B, (0, 2, 1)) if tB else B
+    opA = opA.conj() if cA else opA
+    opB = opB.conj() if cB else opB
+    assert np.abs(out - opA @ opB).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (5, 3), (31, 31), (32, 32), (33, 7), (100, 100),
+                                    (450, 450)])
+def test_zgesv(gpu_ctx, n, nrhs):
+    A, B = crand(2, n, n), crand(2, n, nrhs)
+    X, info = g.zgesv(gpu_ctx, A, B)
+    assert not info.any()
+    ref = np.linalg.solve(A, B)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-11
+    assert np.abs(A @ X - B).max() / (np.abs(A).max() * np.abs(X).max() * n) < 1e-14
+
+
+def test_zgesv_singular_flag(gpu_ctx):
+    A = crand(1, 8, 8)
+    A[0, :, 3] = 0.0
+    _, info = g.zgesv(gpu_ctx, A, crand(1, 8, 2))
+    assert info[0] & 4
+
+
+def match_eigenvalues(a, b):
+    b = list(b)
+    worst = 0.0
+    for x in a:
+        j = int(np.argmin(np.abs(np.array(b) - x)))
+        worst = max(worst, abs(b[j] - x))
+        b.pop(j)
+    return worst
+
+
+@pytest.mark.parametrize("n", [2, 3, 10, 50, 98])
+def test_zgeev_random(gpu_ctx, n):
+    A = crand(2, n, n)
+    lam, W, resid, info = g.zgeev(gpu_ctx, A)
+    assert not info.any()
+    for b in range(2):
+        ref = np.linalg.eigvals(A[b])
+        assert match_eigenvalues(lam[b], ref) < 1e-11 * np.abs(ref).max()
+        assert resid[b] < 1e-12
+        assert np.abs(np.linalg.norm(W[b], axis=0) - 1).max() < 1e-13
+        r = np.abs(A[b] @ W[b] - W[b] * lam[b]).max() / np.abs(A[b]).max()
+        assert abs(r - resid[b]) < 1e-13
+
+
+def rcwa_pq(N, seed):
+    r = np.random.default_rng(seed)
+    Lp = 400e-9
+    fx, fy = r.uniform(0.2, 0.8, 2)
+    e1 = complex(r.uniform(2, 12), r.uniform(0, 0.1))
+    xe, ye = [0, fx * Lp, Lp], [0, fy * Lp, Lp]
+    eps = np.array([[e1, 1.0], [1.0, 1.0]])
+    lay = o.Layer(xe, ye, eps)
+    s = o.RCWASolver(N, N)
+    s.disable_pml()
+    s.add_layer(lay)
+    lam = r.uniform(400e-9, 700e-9)
+    k0 = 2 * np.pi / lam
+    th = math.radians(r.uniform(0, 35))
+    s.set_bloch_wavevector(k0 * math.sin(th), 0.0)
+    a, b = s.alpha_beta()
+    P, Q = lay.wave_eqn_coeff(np.diag(a / k0), np.diag(b / k0), 2 * np.pi, 2 * N + 1, 2 * N + 1)
+    return P @ Q
+
+
+@pytest.mark.parametrize("N", [1, 3, 7])
+def test_zgeev_rcwa_matrices(gpu_ctx, N):
+    A = np.stack([rcwa_pq(N, s) for s in range(3)])
+    lam, W, resid, info = g.zgeev(gpu_ctx, A)
+    assert not info.any()
+    for b in range(3):
+        ref = np.linalg.eigvals(A[b])
+        assert match_eigenvalues(lam[b], ref) < 1e-10 * np.abs(ref).max()
+        assert resid[b] < 1e-11
+
+
+def test_zgeev_degenerate_block(gpu_ctx):
+    # exactly degenerate pairs (a homogeneous layer's PQ is diag(kz^2) x 2)
+    d = np.repeat(crand(20), 2)
+    Q, _ = np.linalg.qr(crand(40, 40))
+    A = (Q @ np.diag(d) @ np.linalg.inv(Q))[None]
+    lam, W, resid, info = g.zgeev(gpu_ctx, A)
+    assert not info.any() and resid[0] < 1e-11
+    assert np.linalg.cond(W[0]) < 1e6
+
+
+@pytest.mark.parametrize("nX,nY,xe,ye", [
+    (5, 0, [0.0, 1.5e-7, 3e-7], [0.0, 3e-7]),
+    (3, 2, [0.0, 1e-7, 2.2e-7, 4e-7], [0.0, 1.7e-7, 4e-7]),
+    (7, 7, [0.0, 0.9e-7, 2e-7, 3.1e-7, 4e-7], [0.0, 1.2e-7, 2e-7, 2.9e-7, 4e-7]),
+])
+def test_convolution_matrices(gpu_ctx, nX, nY, xe, ye):
+    eps = (rng.uniform(1, 12, (len(ye) - 1, len(xe) - 1)) +
+           1j * rng.uniform(0, 0.2, (len(ye) - 1, len(xe) - 1)))
+    got = g.convolution(gpu_ctx, nX, nY, xe, ye, eps)
+    ref = o.convolution_matrices(xe, ye, eps, 2 * nX + 1, 2 * nY + 1)
+    for a, b in zip(got, ref):
+        assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()

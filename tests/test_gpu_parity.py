@@ -1,0 +1,152 @@
+"""End-to-end parity of the HIP engine with the FP64 oracle (per-order R/T,
+both polarisations), plus the reference's published values and closed forms
+through the GPU path.  Tolerances: 1e-9 absolute on per-order efficiencies for
+the small cases (FP64 everywhere; the bar of the path is 1e-6), published
+values to their printed 6 decimals."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import rcwa_oracle as o
+from openrcwa_amd import engine, ofd, workloads
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "published_values.json")
+TOL = 1e-9
+
+
+def L(e1, e2=None, fill=0.5, d=0.0, e1i=0.0, e2i=0.0, fy=1.0):
+    e2 = e1 if e2 is None else e2
+    return dict(eps1=e1, eps2=e2, eps1i=e1i, eps2i=e2i, fill=fill, fill_y=fy, thickness=d)
+
+
+def compare_ofd(ctx, nX, period, layers, freqs, theta=0.0, phi=0.0, nY=0, period_y=None,
+                poles=None, tol=TOL):
+    gpu = ofd.solve_ofd(ctx, nX, period, layers, freqs, theta, phi, nY, period_y, poles)
+    assert not gpu["status"].any(), gpu["status"]
+    worst = 0.0
+    for f, freq in enumerate(np.atleast_1d(freqs)):
+        ref = o.ofd_solve_frequency(nX, period, layers, freq, theta, phi, nY, period_y, poles)
+        for p in range(2):
+            worst = max(worst, abs(gpu["R"][f, p] - ref["R"][p]), abs(gpu["T"][f, p] - ref["T"][p]),
+                        np.abs(gpu["REF"][f, p] - ref["REF"][p]).max(),
+                        np.abs(gpu["TRN"][f, p] - ref["TRN"][p]).max())
+    assert worst < tol, worst
+    return gpu
+
+
+def test_grating_1d_sweep(gpu_ctx):
+    """The 1D binary grating (data/sample/grating.ofd, N=5, 16 frequencies)."""
+    w = workloads.grating1d(nX=5, n_freq=16)
+    compare_ofd(gpu_ctx, w["nX"], w["period"], w["layers"], w["freqs"])
+
+
+def test_published_values_on_gpu(gpu_ctx):
+    gd = json.load(open(GOLDEN))
+    gr = gd["grating"]
+    for c in gr["cases"]:
+        r = ofd.solve_ofd(gpu_ctx, c["N"], gr["period"], gr["layers"], [gr["freq"]])
+        assert abs(r["R"][0, 0] - c["R_TE"]) < 5.1e-7
+        assert abs(r["R"][0, 1] - c["R_TM"]) < 5.1e-7
+    m = gd["metal"]
+    r = ofd.solve_ofd(gpu_ctx, m["N"], m["period"], m["layers"], [m["freq"]])
+    assert abs(r["R"][0, 0] - m["R"]) < 5.1e-7 and abs(r["T"][0, 0] - m["T"]) < 5.1e-7
+    b = gd["oblique"]
+    r = ofd.solve_ofd(gpu_ctx, b["N"], b["period"], b["layers"], [b["freq"]], theta=b["theta"])
+    assert abs(r["R"][0, 0] - b["R_TE"]) < 5.1e-7 and r["R"][0, 1] < b["R_TM_max"]
+
+
+@pytest.mark.parametrize("theta,phi", [(0, 0), (30, 0), (56.30993247, 0), (70, 0), (45, 37)])
+def test_fresnel_on_gpu(gpu_ctx, theta, phi):
+    r = ofd.solve_ofd(gpu_ctx, 2, 4e-7, [L(1.0), L(2.25)], [5e14], theta, phi)
+    ti = math.radians(theta)
+    st = math.sin(ti) / 1.5
+    ci, ct = math.cos(ti), math.sqrt(1 - st * st)
+    rs = ((ci - 1.5 * ct) / (ci + 1.5 * ct)) ** 2
+    rp = ((1.5 * ci - ct) / (1.5 * ci + ct)) ** 2
+    assert abs(r["R"][0, 0] - rs) < 1e-12 and abs(r["R"][0, 1] - rp) < 1e-12
+    assert np.abs(r["R"] + r["T"] - 1).max() < 1e-12
+
+
+def test_oblique_conical_grating(gpu_ctx):
+    grat = [L(1.0), L(4.0, 1.0, fill=0.4, d=2.5e-7), L(2.25)]
+    for th, ph in ((30, 20), (40, 30), (-25, 0), (25, 180)):
+        g = compare_ofd(gpu_ctx, 8, 4e-7, grat, np.linspace(4.3e14, 7.5e14, 3), th, ph)
+        assert np.abs(g["R"] + g["T"] - 1).max() < 1e-9
+
+
+def test_dispersive_and_lossy(gpu_ctx):
+    layers = [L(1.0), L(1.0, d=3e-8), L(2.25)]
+    poles = {(1, 1): [(1.0, 1.36e16, 3.0e13, 0.0)], (1, 2): [(1.0, 1.36e16, 3.0e13, 0.0)]}
+    compare_ofd(gpu_ctx, 2, 5e-7, layers, np.linspace(4e14, 7e14, 7), 30.0, 0.0, poles=poles)
+    lor = {(1, 1): [(2.0, 3.0e15, 2.0e14, 4.0e15)]}
+    g1 = [L(1.0), L(1.0, 2.25, fill=0.3, d=1e-7), L(2.25)]
+    compare_ofd(gpu_ctx, 6, 5e-7, g1, np.linspace(4e14, 7e14, 4), 20.0, 0.0, poles=lor)
+
+
+def test_2d_pillars_small(gpu_ctx):
+    lay = [L(1.0), L(2.25, 1.0, fill=0.6, d=2e-7, fy=0.6), L(2.25)]
+    g = compare_ofd(gpu_ctx, 3, 5e-7, lay, np.linspace(3.331e14, 4.835e14, 3), nY=3,
+                    period_y=5e-7)
+    assert np.abs(g["R"][:, 0] - g["R"][:, 1]).max() < 1e-9          # C4: TE = TM
+    lay2 = [L(1.0), L(2.25 + 0.05j, 1.0, fill=0.55, d=1.5e-7, fy=0.35),
+            L(6.0, 1.0, fill=0.3, d=0.8e-7, fy=0.7), L(2.25)]
+    compare_ofd(gpu_ctx, 2, 4e-7, lay2, [5.5e14, 6.5e14], 20.0, 45.0, nY=3, period_y=4.5e-7)
+
+
+def pillar_subset(nX, idx):
+    w = workloads.pillar2d(nX=nX, nY=nX)
+    return w, np.asarray(idx)
+
+
+def gpu_pillar(ctx, w, idx):
+    return engine.solve_batch(ctx, w["nX"], w["nY"], w["coord_x"], w["coord_y"], w["thickness"],
+                              w["lam"][idx], w["kx0"][idx], w["ky0"][idx], w["eps"],
+                              w["pols"][idx])
+
+
+def oracle_pillar(w, i):
+    th, ph = w["angles"][i]
+    return o.solve_stack(w["nX"], w["nY"], w["period"], w["period"], w["coord_x"], w["coord_y"],
+                         list(w["eps"]), w["thickness"], w["lam"][i], th, ph)
+
+
+@pytest.mark.parametrize("nX", [2, 4])
+def test_pillar_workload_reduced_orders(gpu_ctx, nX):
+    w, idx = pillar_subset(nX, [0, 1, 3, 6, 7, 1001, 2047])
+    res = gpu_pillar(gpu_ctx, w, idx)
+    assert not res.status.any()
+    for j, i in enumerate(idx):
+        REF, TRN, _ = oracle_pillar(w, i)
+        for p in range(2):
+            assert np.abs(res.ref[j, p] - REF[p]).max() < TOL
+            assert np.abs(res.trn[j, p] - TRN[p]).max() < TOL
+
+
+def test_pillar_workload_full_size(gpu_ctx):
+    """The 2D 15x15-order pillar workload at its real size (n = 450) on a
+    handful of instances; the north-star bar is max |dR|, |dT| <= 1e-6."""
+    w, idx = pillar_subset(7, [0, 5, 1234])
+    res = gpu_pillar(gpu_ctx, w, idx)
+    assert not res.status.any()
+    worst = 0.0
+    for j, i in enumerate(idx):
+        REF, TRN, _ = oracle_pillar(w, i)
+        for p in range(2):
+            worst = max(worst, np.abs(res.ref[j, p] - REF[p]).max(),
+                        np.abs(res.trn[j, p] - TRN[p]).max())
+    assert worst < 1e-8, worst
+
+
+def test_pillar_full_size_properties(gpu_ctx):
+    """Size-independent checks over a larger slice of the full workload:
+    lossy stack => 0 <= R, T and R + T <= 1; sum of orders == totals."""
+    w, idx = pillar_subset(7, np.arange(0, 2048, 16))
+    res = gpu_pillar(gpu_ctx, w, idx)
+    assert not res.status.any()
+    tot = res.R + res.T
+    assert (res.ref > -1e-12).all() and (res.trn > -1e-12).all()
+    assert (tot < 1 + 1e-9).all() and (tot > 0.3).all()

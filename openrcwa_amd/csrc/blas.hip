@@ -12,8 +12,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // GEMM: 64x64 output tile per workgroup (4 waves, 32x32 each = 2x2 MFMA
 // 16x16 sub-tiles), K staged through LDS 16 at a time, re/im split into
 // separate LDS planes so each MFMA reads one double per lane.
-// MFMA f64 16x16x4 lane layout: A[i][k] in lane i+16k, B[k][j] in lane
-// j+16k, D[i][j] in lane j+16(i/4), element i%4.
+// MFMA f64 16x16x4 lane layout (measured on gfx950): A[i][k] in lane i+16k,
+// B[k][j] in lane j+16k, D[i][j] in lane j+16(i%4), element i/4.
 // ---------------------------------------------------------------------------
 template <bool BETA0>
 __global__ __launch_bounds__(256) void k_zgemm(
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_zgemm(
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int gm = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
         const int gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N) {
           cd v = alpha * mk(accR[i][j][r], accI[i][j][r]);

@@ -29,7 +29,7 @@ def _p(a):
 
 def fortran_batch(mats):
     """(batch, rows, cols) complex -> contiguous column-major per matrix."""
-    return np.ascontiguousarray(np.transpose(np.asarray(mats, dtype=complex), (0, 2, 1)))
+    return np.array(np.transpose(np.asarray(mats, dtype=complex), (0, 2, 1)), order="C", copy=True)
 
 
 def from_fortran(buf, rows, cols):

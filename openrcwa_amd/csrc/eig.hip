@@ -18,6 +18,8 @@
 
 namespace orcwa {
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 static constexpr double kUlp = DBL_EPSILON;
 static constexpr double kSafmin = DBL_MIN;
 
@@ -338,6 +340,521 @@ __global__ __launch_bounds__(256) void k_qr_simple(int n, cd* const* Hp, cd* con
 }
 
 // ---------------------------------------------------------------------------
+// 3b. windowed small-bulge multishift QR (one workgroup per matrix).
+//
+// A sweep chases QNS single-shift bulges (2x2 unitaries G, as above) as a
+// train through the active block [l, hi].  The chase runs inside a QW x QW
+// window held in LDS; the window's rotations are accumulated into a QW x QW
+// unitary U (LDS) and applied afterwards, with FP64 MFMA, to the rows right of
+// the window, the columns above it and the Schur vectors Z.  A bulge may
+// process pair p only when the bulge ahead of it has already processed
+// p + 2 or later; with that spacing the interleaved train is the same
+// similarity as QNS consecutive single-shift sweeps.  Blocks of at most QW
+// rows are finished entirely in LDS.  Shifts are the eigenvalues of the
+// trailing QNS x QNS submatrix.
+// ---------------------------------------------------------------------------
+#ifdef ORCWA_QR_PROFILE
+// per-phase shader-clock totals of matrix 0 (profiling builds only)
+__device__ unsigned long long g_qr_prof[16];
+#define QPROF_T0(v) unsigned long long v = (threadIdx.x == 0 && blockIdx.x == 0) ? clock64() : 0
+#define QPROF_ADD(slot, v) \
+  if (threadIdx.x == 0 && blockIdx.x == 0) g_qr_prof[slot] += clock64() - (v)
+#define QPROF_CNT(slot) \
+  if (threadIdx.x == 0 && blockIdx.x == 0) g_qr_prof[slot] += 1
+#else
+#define QPROF_T0(v)
+#define QPROF_ADD(slot, v)
+#define QPROF_CNT(slot)
+#endif
+
+constexpr int QW = 64;
+constexpr int QLD = QW + 1;   // padded LDS leading dimension (bank spread)
+constexpr int QNS = 16;
+constexpr int QSCHED = QNS * QW;
+
+struct QrShared {
+  cd Hw[QLD * QLD];     // window / small block (column-major, ld QLD)
+  cd U[QLD * QLD];      // accumulated unitary; trailing block for the shifts
+  short sched_b[QSCHED];
+  short sched_p[QSCHED];
+  short pass_start[QSCHED + 1];
+  cd rot_c[QNS], rot_s[QNS];
+  int rot_r[QNS];
+  cd shifts[QNS];
+  int bp[QNS];         // next pair of each bulge (-1: not introduced)
+  int bdone[QNS];
+  cd c, s, sh;
+  int nsched, npass, l, ok, ired[4];
+};
+
+__device__ inline void lds_rot_left(cd* B, int r, int j0, int j1, cd c, cd s) {
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += 256) {
+    const cd a = B[r + j * QLD], b = B[r + 1 + j * QLD];
+    B[r + j * QLD] = cmul(c, a) + cmul(s, b);
+    B[r + 1 + j * QLD] = c * b - s * a;
+  }
+}
+__device__ inline void lds_rot_right(cd* B, int col, int i0, int i1, cd c, cd s) {
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+    const cd p = B[i + col * QLD], q = B[i + (col + 1) * QLD];
+    B[i + col * QLD] = p * c + q * s;
+    B[i + (col + 1) * QLD] = q * conj(c) - p * conj(s);
+  }
+}
+
+__device__ inline bool small_subdiag_lds(const cd* B, int k, int lo, int hi, double smlnum) {
+  const cd h = B[k + (k - 1) * QLD];
+  const double ah = cabs1(h);
+  if (ah <= smlnum) return true;
+  const cd hkk = B[k + k * QLD], hk1 = B[(k - 1) + (k - 1) * QLD];
+  double tst = cabs1(hk1) + cabs1(hkk);
+  if (tst == 0.0) {
+    if (k - 2 >= lo) tst += cabs1(B[(k - 1) + (k - 2) * QLD]);
+    if (k + 1 <= hi) tst += cabs1(B[(k + 1) + k * QLD]);
+  }
+  if (ah <= kUlp * tst) {
+    const double hup = cabs1(B[(k - 1) + k * QLD]);
+    const double ab = fmax(ah, hup), ba = fmin(ah, hup);
+    const double d2 = cabs1(hk1 - hkk);
+    const double aa = fmax(cabs1(hkk), d2), bb = fmin(cabs1(hkk), d2);
+    const double ss = aa + ab;
+    if (ba * (ab / ss) <= fmax(smlnum, kUlp * (bb * (aa / ss)))) return true;
+  }
+  return false;
+}
+
+__device__ inline cd wilkinson_lds(const cd* B, int hi) {
+  cd t = B[hi + hi * QLD];
+  cd u = csqrt_principal(B[(hi - 1) + hi * QLD]) * csqrt_principal(B[hi + (hi - 1) * QLD]);
+  double s = cabs1(u);
+  if (s != 0.0) {
+    cd x = 0.5 * (B[(hi - 1) + (hi - 1) * QLD] - t);
+    double sx = cabs1(x);
+    s = fmax(s, sx);
+    cd xs = (1.0 / s) * x, us = (1.0 / s) * u;
+    cd y = s * csqrt_principal(xs * xs + us * us);
+    if (sx > 0.0) {
+      cd xx = (1.0 / sx) * x;
+      if (xx.re * y.re + xx.im * y.im < 0.0) y = -y;
+    }
+    t = t - u * cdiv(u, x + y);
+  }
+  return t;
+}
+
+// Single-shift QR of the m x m (m <= 64) Hessenberg block B (LDS) to upper
+// triangular form, accumulating the rotations into U when wantu.  Run by
+// wave 0 only: every lane computes the (uniform) deflation point, shift and
+// rotation redundantly, lane j owns column j / row j of the updates, and LDS
+// ordering inside the wave is enforced with wavefront-scope fences.
+__device__ inline void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bool wave_small_qr(cd* B, cd* U, int m, bool wantu, int maxit) {
+  const int lane = threadIdx.x & 63;
+  const double smlnum = kSafmin * ((double)m / kUlp);
+  int hi = m - 1, its = 0, tot = 0;
+  while (hi > 0) {
+    const bool flag = (lane >= 1 && lane <= hi) ? small_subdiag_lds(B, lane, 0, hi, smlnum) : false;
+    const unsigned long long mask = __ballot(flag);
+    const int l = mask ? 63 - __clzll(mask) : 0;
+    wave_fence();
+    if (l > 0 && lane == 0) B[l + (l - 1) * QLD] = mk(0.0, 0.0);
+    wave_fence();
+    if (l == hi) { --hi; its = 0; continue; }
+    ++its;
+    if (++tot > maxit) return false;
+    cd sh;
+    if (its % 30 == 10) sh = mk(0.75 * cabs1(B[(l + 1) + l * QLD]), 0.0) + B[l + l * QLD];
+    else if (its % 30 == 20) sh = mk(0.75 * cabs1(B[hi + (hi - 1) * QLD]), 0.0) + B[hi + hi * QLD];
+    else sh = wilkinson_lds(B, hi);
+    for (int k = l; k < hi; ++k) {
+      cd x0, x1;
+      if (k == l) { x0 = B[l + l * QLD] - sh; x1 = B[(l + 1) + l * QLD]; }
+      else { x0 = B[k + (k - 1) * QLD]; x1 = B[(k + 1) + (k - 1) * QLD]; }
+      const double nr = sqrt(norm2(x0) + norm2(x1));
+      cd c = mk(1.0, 0.0), sn = mk(0.0, 0.0);
+      if (nr != 0.0) {
+        const double inv = 1.0 / nr;
+        c = inv * x0;
+        sn = inv * x1;
+      }
+      wave_fence();
+      if (nr != 0.0 && k > l && lane == 0) {
+        B[k + (k - 1) * QLD] = mk(nr, 0.0);
+        B[(k + 1) + (k - 1) * QLD] = mk(0.0, 0.0);
+      }
+      if (lane >= k && lane < m) {
+        const cd a = B[k + lane * QLD], b = B[k + 1 + lane * QLD];
+        B[k + lane * QLD] = cmul(c, a) + cmul(sn, b);
+        B[k + 1 + lane * QLD] = c * b - sn * a;
+      }
+      if (wantu && lane < m) {
+        const cd p = U[lane + k * QLD], q = U[lane + (k + 1) * QLD];
+        U[lane + k * QLD] = p * c + q * sn;
+        U[lane + (k + 1) * QLD] = q * conj(c) - p * conj(sn);
+      }
+      wave_fence();
+      if (lane <= min(k + 2, hi)) {
+        const cd p = B[lane + k * QLD], q = B[lane + (k + 1) * QLD];
+        B[lane + k * QLD] = p * c + q * sn;
+        B[lane + (k + 1) * QLD] = q * conj(c) - p * conj(sn);
+      }
+      wave_fence();
+    }
+  }
+  return true;
+}
+
+// block-wide wrapper: wave 0 works, the result (ok) is shared
+__device__ __forceinline__ bool lds_small_qr(QrShared& S, cd* B, int m, bool wantu, int maxit) {
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const bool ok = wave_small_qr(B, S.U, m, wantu, maxit);
+    if (threadIdx.x == 0) S.ok = ok ? 1 : 0;
+  }
+  __syncthreads();
+  return S.ok != 0;
+}
+
+// X <- U^H X for rows [a, a+wa) of global X (ld), columns [c0, c1).  Each
+// wave owns 16-column blocks; the block's operand is loaded in two halves of
+// 32 rows whose loads are issued together, so two memory latencies are paid
+// per block.
+__device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long long ld, int a, int wa, int c0,
+                                 int c1) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int HALF = QW / 8;
+  for (int j0 = c0 + 16 * wave; j0 < c1; j0 += 64) {
+    const int j = j0 + (lane & 15);
+    v4d accR[4], accI[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) { accR[t] = (v4d){0, 0, 0, 0}; accI[t] = (v4d){0, 0, 0, 0}; }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      cd cur[HALF];
+#pragma unroll
+      for (int q = 0; q < HALF; ++q) {
+        const int k = 4 * (HALF * h + q) + (lane >> 4);
+        cur[q] = (k < wa && j < c1) ? X[(a + k) + (long long)j * ld] : mk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int q = 0; q < HALF; ++q) {
+        const int k = 4 * (HALF * h + q) + (lane >> 4);
+        const cd bv = cur[q];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // A[i][k] = conj(U[k][i])
+          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, bv.re, accR[t], 0, 0, 0);
+          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.im, bv.im, accR[t], 0, 0, 0);
+          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, bv.im, accI[t], 0, 0, 0);
+          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.im, bv.re, accI[t], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * t + (lane >> 4) + 4 * r;
+        if (i < wa && j < c1) X[(a + i) + (long long)j * ld] = mk(accR[t][r], accI[t][r]);
+      }
+  }
+}
+
+// Y <- Y U for rows [r0, r1) of global Y (ld), columns [a, a+wa); same
+// blocking over 16-row blocks.
+__device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long long ld, int r0, int r1, int a,
+                                int wa) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int HALF = QW / 8;
+  for (int i0 = r0 + 16 * wave; i0 < r1; i0 += 64) {
+    const int i = i0 + (lane & 15);
+    v4d accR[4], accI[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) { accR[t] = (v4d){0, 0, 0, 0}; accI[t] = (v4d){0, 0, 0, 0}; }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      cd cur[HALF];
+#pragma unroll
+      for (int q = 0; q < HALF; ++q) {
+        const int k = 4 * (HALF * h + q) + (lane >> 4);
+        cur[q] = (k < wa && i < r1) ? Y[i + (long long)(a + k) * ld] : mk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int q = 0; q < HALF; ++q) {
+        const int k = 4 * (HALF * h + q) + (lane >> 4);
+        const cd av = cur[q];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // B[k][j] = U[k][j]
+          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, u.re, accR[t], 0, 0, 0);
+          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.im, u.im, accR[t], 0, 0, 0);
+          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, u.im, accI[t], 0, 0, 0);
+          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.im, u.re, accI[t], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ii = i0 + (lane >> 4) + 4 * r;
+        const int jj = 16 * t + (lane & 15);
+        if (ii < r1 && jj < wa) Y[ii + (long long)(a + jj) * ld] = mk(accR[t][r], accI[t][r]);
+      }
+  }
+}
+
+// load H[a:a+wa, a:a+wa] into S.Hw and reset S.U to the identity (zero padded)
+__device__ __forceinline__ void ms_load_window(QrShared& S, const cd* H, int n, int a, int wa) {
+  constexpr int PER = QW * QW / 256 / 2;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    cd v[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = threadIdx.x + 256 * (PER * h + t);
+      const int i = e % QW, j = e / QW;
+      v[t] = (i < wa && j < wa) ? H[(a + i) + (long long)(a + j) * n] : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = threadIdx.x + 256 * (PER * h + t);
+      const int i = e % QW, j = e / QW;
+      S.Hw[i + j * QLD] = v[t];
+      S.U[i + j * QLD] = (i == j && i < wa) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+    }
+  }
+}
+
+__device__ __forceinline__ void ms_store_window(const QrShared& S, cd* H, int n, int a, int wa) {
+  for (int e = threadIdx.x; e < QW * QW; e += 256) {
+    const int i = e % QW, j = e / QW;
+    if (i < wa && j < wa) H[(a + i) + (long long)(a + j) * n] = S.Hw[i + j * QLD];
+  }
+}
+
+// the window's deferred updates: rows right, columns above, Schur vectors
+__device__ __forceinline__ void ms_apply_window(const QrShared& S, cd* H, cd* Z, int n, int a, int wa) {
+  if (a + wa < n) ms_apply_uh_rows(S, H, n, a, wa, a + wa, n);
+  if (a > 0) ms_apply_u_cols(S, H, n, 0, a, a, wa);
+  ms_apply_u_cols(S, Z, n, 0, n, a, wa);
+}
+
+__global__ __launch_bounds__(256) void k_qr_ms(int n, cd* const* Hp, cd* const* Zp, int* info,
+                                               int maxsweep) {
+  __shared__ QrShared S;
+  __shared__ int s_red[256];
+  const int b = blockIdx.x;
+  cd* __restrict__ H = Hp[b];
+  cd* __restrict__ Z = Zp[b];
+  const int tid = threadIdx.x;
+  const double smlnum = kSafmin * ((double)n / kUlp);
+  int hi = n - 1, sweeps = 0, noprog = 0;
+  while (hi >= 0) {
+    QPROF_T0(tp0);
+    int best = 0;
+    for (int k = 1 + tid; k <= hi; k += 256)
+      if (small_subdiag(H, n, k, 0, hi, smlnum)) best = max(best, k);
+    s_red[tid] = best;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (tid < st) s_red[tid] = max(s_red[tid], s_red[tid + st]);
+      __syncthreads();
+    }
+    const int l = s_red[0];
+    __syncthreads();
+    if (l > 0 && tid == 0) H[l + (long long)(l - 1) * n] = mk(0.0, 0.0);
+    __syncthreads();
+    QPROF_ADD(0, tp0);
+    if (l == hi) { --hi; noprog = 0; continue; }
+    const int m = hi - l + 1;
+    if (m <= QW) {
+      QPROF_T0(tp1);
+      QPROF_CNT(8);
+      ms_load_window(S, H, n, l, m);
+      __syncthreads();
+      const bool ok = lds_small_qr(S, S.Hw, m, true, 30 * m + 100);
+      ms_store_window(S, H, n, l, m);
+      __syncthreads();
+      ms_apply_window(S, H, Z, n, l, m);
+      __syncthreads();
+      QPROF_ADD(1, tp1);
+      if (!ok) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
+      hi = l - 1;
+      noprog = 0;
+      continue;
+    }
+    QPROF_T0(tp2);
+    // shifts: eigenvalues of the trailing QNS x QNS block (solved in S.U)
+    constexpr int ns = QNS;
+    for (int e = tid; e < QW * QW; e += 256) {
+      const int i = e % QW, j = e / QW;
+      cd v = mk(0.0, 0.0);
+      if (i < ns && j < ns && i <= j + 1)
+        v = H[(hi - ns + 1 + i) + (long long)(hi - ns + 1 + j) * n];
+      S.U[i + j * QLD] = v;
+    }
+    __syncthreads();
+    lds_small_qr(S, S.U, ns, false, 30 * ns + 100);
+    ++noprog;
+    if (tid < ns) {
+      cd sh = S.U[tid + tid * QLD];
+      if (noprog % 6 == 0) {
+        const double pert = 0.75 * cabs1(H[hi + (long long)(hi - 1) * n]);
+        double sn, cs;
+        sincos(2.0 * 3.14159265358979 * tid / ns, &sn, &cs);
+        sh = sh + mk(pert * cs, pert * sn);
+      }
+      S.shifts[tid] = sh;
+      S.bp[tid] = -1;
+      S.bdone[tid] = 0;
+    }
+    __syncthreads();
+    QPROF_ADD(2, tp2);
+    QPROF_CNT(9);
+    if (++sweeps > maxsweep) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
+    int a = l;
+    while (true) {
+      QPROF_T0(tp3);
+      QPROF_CNT(10);
+      const int bend = min(a + QW, hi + 1);
+      const int wa = bend - a;
+      ms_load_window(S, H, n, a, wa);
+#ifdef ORCWA_QR_PROFILE
+      __syncthreads();
+      QPROF_ADD(6, tp3);
+#endif
+      if (tid < 64) {
+        // Schedule of (bulge, pair) moves inside this window, grouped in
+        // passes (the moves of one pass touch disjoint row and column pairs).
+        // Lane i owns bulge i.  Within a pass bulge i moves iff it can move
+        // on its own (A) or it may follow (B) and bulge i-1 moved: a boolean
+        // carry chain solved on ballot masks.  A follower's spacing to its
+        // predecessor is >= 2 pairs, so "predecessor moved" always suffices.
+        const int lane = tid;
+        const int lim = (bend - 1 < hi) ? bend - 3 : hi - 1;
+        int bpv = (lane < QNS) ? S.bp[lane] : 0;
+        int bdv = (lane < QNS) ? S.bdone[lane] : 1;
+        int cnt = 0, np = 0;
+        while (cnt + QNS <= QSCHED) {
+          const int pbp = __shfl_up(bpv, 1, 64), pbd = __shfl_up(bdv, 1, 64);
+          const bool first = (lane == 0);
+          const bool intro = (bpv < 0);
+          const bool own = !bdv && (intro ? (a == l) : (bpv <= lim && bpv >= a + 1));
+          bool A, B;
+          if (intro) {
+            A = first || pbd || pbp >= l + 3;
+            B = !first && pbp >= 0 && pbp + 1 >= l + 3;
+          } else {
+            A = first || pbd || bpv <= pbp - 3;
+            B = true;
+          }
+          const unsigned long long mA = __ballot(own && A), mB = __ballot(own && B);
+          unsigned long long mv = mA;
+#pragma unroll
+          for (int it = 0; it < QNS; ++it) mv = mA | (mB & (mv << 1));
+          if (mv == 0ull) break;
+          if ((mv >> lane) & 1ull) {
+            const int pos = cnt + __popcll(mv & ((1ull << lane) - 1ull));
+            const int pp = intro ? l : bpv;
+            S.sched_b[pos] = (short)(intro ? -(lane + 1) : lane + 1);
+            S.sched_p[pos] = (short)pp;
+            bpv = pp + 1;
+            if (bpv >= hi) bdv = 1;
+          }
+          if (lane == 0) S.pass_start[np] = (short)cnt;
+          cnt += __popcll(mv);
+          ++np;
+        }
+        int cand = (lane < QNS && !bdv) ? (bpv < 0 ? l + 1 : bpv) : (1 << 30);
+        for (int off = 32; off > 0; off >>= 1) cand = min(cand, __shfl_xor(cand, off, 64));
+        if (lane < QNS) { S.bp[lane] = bpv; S.bdone[lane] = bdv; }
+        if (lane == 0) {
+          S.pass_start[np] = (short)cnt;
+          S.nsched = cnt;
+          S.npass = np;
+          S.l = cand;
+        }
+      }
+      __syncthreads();
+      QPROF_ADD(3, tp3);
+      QPROF_T0(tp4);
+      const int cnt = S.nsched, npass = S.npass;
+      const int rlim = min(hi - a + 1, wa);
+      for (int ps = 0; ps < npass; ++ps) {
+        QPROF_CNT(11);
+        const int e0 = S.pass_start[ps], ne = S.pass_start[ps + 1] - e0;
+        if (tid < ne) {
+          const int e = e0 + tid;
+          const int r = S.sched_p[e] - a;
+          const int code = S.sched_b[e];
+          cd x0, x1;
+          if (code < 0) { x0 = S.Hw[r + r * QLD] - S.shifts[-code - 1]; x1 = S.Hw[(r + 1) + r * QLD]; }
+          else { x0 = S.Hw[r + (r - 1) * QLD]; x1 = S.Hw[(r + 1) + (r - 1) * QLD]; }
+          const double nr = sqrt(norm2(x0) + norm2(x1));
+          cd c = mk(1.0, 0.0), sn = mk(0.0, 0.0);
+          if (nr != 0.0) {
+            const double inv = 1.0 / nr;
+            c = inv * x0;
+            sn = inv * x1;
+            if (code > 0) { S.Hw[r + (r - 1) * QLD] = mk(nr, 0.0); S.Hw[(r + 1) + (r - 1) * QLD] = mk(0.0, 0.0); }
+          }
+          S.rot_c[tid] = c;
+          S.rot_s[tid] = sn;
+          S.rot_r[tid] = r;
+        }
+        __syncthreads();
+        // all left rotations (disjoint row pairs) and the U columns
+        for (int idx = tid; idx < ne * QW; idx += 256) {
+          const int q = idx / QW, j = idx % QW;
+          const int r = S.rot_r[q];
+          const cd c = S.rot_c[q], sn = S.rot_s[q];
+          if (j >= r && j < wa) {
+            const cd x = S.Hw[r + j * QLD], y = S.Hw[r + 1 + j * QLD];
+            S.Hw[r + j * QLD] = cmul(c, x) + cmul(sn, y);
+            S.Hw[r + 1 + j * QLD] = c * y - sn * x;
+          }
+          if (j < wa) {
+            const cd pu = S.U[j + r * QLD], qu = S.U[j + (r + 1) * QLD];
+            S.U[j + r * QLD] = pu * c + qu * sn;
+            S.U[j + (r + 1) * QLD] = qu * conj(c) - pu * conj(sn);
+          }
+        }
+        __syncthreads();
+        // all right rotations (disjoint column pairs)
+        for (int idx = tid; idx < ne * QW; idx += 256) {
+          const int q = idx / QW, i = idx % QW;
+          const int r = S.rot_r[q];
+          if (i < min(r + 3, rlim)) {
+            const cd c = S.rot_c[q], sn = S.rot_s[q];
+            const cd pv = S.Hw[i + r * QLD], qv = S.Hw[i + (r + 1) * QLD];
+            S.Hw[i + r * QLD] = pv * c + qv * sn;
+            S.Hw[i + (r + 1) * QLD] = qv * conj(c) - pv * conj(sn);
+          }
+        }
+        __syncthreads();
+      }
+      QPROF_ADD(4, tp4);
+      QPROF_T0(tp5);
+      ms_store_window(S, H, n, a, wa);
+      ms_apply_window(S, H, Z, n, a, wa);
+      __syncthreads();
+      QPROF_ADD(5, tp5);
+      const int next = S.l;
+      __syncthreads();
+      if (next == (1 << 30)) break;
+      if (cnt == 0) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
+      a = next - 1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 4. eigenvectors of the upper triangular T (ztrevc mathematics, right
 //    vectors): X upper triangular with unit diagonal.
 // ---------------------------------------------------------------------------
@@ -437,6 +954,17 @@ __global__ void k_zero_below_subdiag(int n, cd* const* Ap) {
   }
 }
 
+#ifdef ORCWA_QR_PROFILE
+extern "C" int orcwa_prof_qr(double* out) {
+  unsigned long long h[16];
+  if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_qr_prof), sizeof(h)) != hipSuccess) return 1;
+  for (int i = 0; i < 16; ++i) out[i] = (double)h[i];
+  unsigned long long z[16] = {0};
+  if (hipMemcpyToSymbol(HIP_SYMBOL(g_qr_prof), z, sizeof(z)) != hipSuccess) return 1;
+  return 0;
+}
+#endif
+
 void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, cd* const* W,
                    cd* lam, const EigWorkspace& ws, int* info, double* resid,
                    cd* const* scratch, int batch, hipEvent_t* ev) {
@@ -452,7 +980,7 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
   if (ev) (void)hipEventRecord(ev[0], s);
   // zero everything below the first subdiagonal (the Householder vectors)
   hipLaunchKernelGGL(k_zero_below_subdiag, dim3(64, batch), dim3(256), 0, s, n, A);
-  hipLaunchKernelGGL(k_qr_simple, dim3(batch), dim3(256), 0, s, n, A, ws.Z, info, 60 * n);
+  hipLaunchKernelGGL(k_qr_ms, dim3(batch), dim3(256), 0, s, n, A, ws.Z, info, 40 + 3 * n);
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(256), 0, s, n, Ac, ws.X);
   zgemm_batched(s, n, n, n, mk(1.0, 0.0), (const cd* const*)ws.Z, 0, 1, n, 0,

@@ -69,7 +69,7 @@ def match_eigenvalues(a, b):
     return worst
 
 
-@pytest.mark.parametrize("n", [2, 3, 10, 50, 98])
+@pytest.mark.parametrize("n", [2, 3, 10, 50, 64, 65, 66, 98, 130, 200])
 def test_zgeev_random(gpu_ctx, n):
     A = crand(2, n, n)
     lam, W, resid, info = g.zgeev(gpu_ctx, A)

@@ -195,6 +195,83 @@ __global__ __launch_bounds__(256) void k_formq(int n, const cd* const* Ap, const
 }
 
 // ---------------------------------------------------------------------------
+// 2b. blocked Q: Q = H_0 ... H_{n-3} = Q_0 Q_1 ... with Q_b = I - V_b T_b V_b^H
+//     (compact WY, zlarft forward columnwise), accumulated backward with three
+//     MFMA GEMMs per block.  V_b is extracted from the reflectors stored below
+//     the subdiagonal by k_hess.
+// ---------------------------------------------------------------------------
+constexpr int QB = 32;
+
+// Vall[:, j] = reflector j as a dense column (zeros above row j+1, 1 at j+1)
+__global__ void k_extract_v(int n, const cd* const* Ap, cd* const* Vp) {
+  const cd* A = Ap[blockIdx.y];
+  cd* V = Vp[blockIdx.y];
+  const long long cnt = (long long)n * (n - 2);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < cnt;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    cd v = mk(0.0, 0.0);
+    if (i == j + 1) v = mk(1.0, 0.0);
+    else if (i > j + 1) v = A[i + (long long)j * n];
+    V[e] = v;
+  }
+}
+
+// T factor of reflector block b (columns b*QB .. b*QB+nbk-1 of Vall):
+//   T[j,j] = tau_j,  T[0:j, j] = -tau_j T[0:j, 0:j] (V^H v_j)[0:j]
+// one workgroup per (block, matrix); T stored QB x QB (ld QB) per block.
+__global__ __launch_bounds__(256) void k_larft(int n, const cd* const* Vp, const cd* tau_all,
+                                               cd* const* Tp) {
+  __shared__ cd G[QB][QB + 1];
+  __shared__ cd Tl[QB][QB + 1];
+  const int blk = blockIdx.x, b = blockIdx.y;
+  const int nref = n - 2;
+  const int j0 = blk * QB;
+  const int nbk = min(QB, nref - j0);
+  const cd* V = Vp[b];
+  const cd* tau = tau_all + (long long)b * n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // G = V_b^H V_b (upper part needed), rows from j0+1 (above: zeros)
+  for (int pr = wave; pr < nbk * nbk; pr += 4) {
+    const int r = pr / nbk, c = pr % nbk;
+    if (r > c) continue;
+    const cd* vr = V + (long long)(j0 + r) * n;
+    const cd* vc = V + (long long)(j0 + c) * n;
+    double ar = 0.0, ai = 0.0;
+    for (int i = j0 + r + 1 + lane; i < n; i += 64) {
+      const cd z = cmul(vr[i], vc[i]);
+      ar += z.re;
+      ai += z.im;
+    }
+    ar = wave_sum(ar);
+    ai = wave_sum(ai);
+    if (lane == 0) G[r][c] = mk(ar, ai);
+  }
+  __syncthreads();
+  if (tid < 64) {
+    for (int j = 0; j < nbk; ++j) {
+      const cd tj = tau[j0 + j];
+      // lane i < j: T[i, j] = -tau_j sum_{k=i}^{j-1} T[i,k] G[k,j]
+      cd acc = mk(0.0, 0.0);
+      if (lane < j)
+        for (int k = lane; k < j; ++k) acc += Tl[lane][k] * G[k][j];
+      if (lane < j) Tl[lane][j] = -(tj * acc);
+      if (lane == j) Tl[j][j] = tj;
+      if (lane > j && lane < QB) Tl[lane][j] = mk(0.0, 0.0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  __syncthreads();
+  cd* T = Tp[b] + (long long)blk * QB * QB;
+  for (int e = tid; e < QB * QB; e += 256) {
+    const int r = e % QB, c = e / QB;
+    T[e] = (r < nbk && c < nbk) ? Tl[r][c] : mk(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 3. single-shift complex QR (zlahqr mathematics; wantt, wantz), one
 //    workgroup per matrix.  Each 2x2 unitary G = [[c, -conj(s)], [s, conj(c)]]
 //    with (c, s) = x/||x|| maps x to (||x||, 0) under G^H.
@@ -972,8 +1049,36 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
   const cd* const* Ac = (const cd* const*)A;
   if (n >= 3) {
     hipLaunchKernelGGL(k_hess, dim3(batch), dim3(256), 3 * n * sizeof(cd), s, n, A, ws.tau);
-    hipLaunchKernelGGL(k_formq, dim3(batch), dim3(256), 2 * n * sizeof(cd), s, n, Ac,
-                       (const cd*)ws.tau, ws.Z);
+    // blocked explicit Q into Z; X holds the dense reflectors, W the T factors
+    // and the two GEMM intermediates until the eigenvectors are formed (W must
+    // hold nblk QB^2 + 2 QB n entries: n >= 104 or so, else the unblocked loop)
+    const int nref = n - 2, nblk = (nref + QB - 1) / QB;
+    if ((long long)nblk * QB * QB + 2LL * QB * n > (long long)n * n) {
+      hipLaunchKernelGGL(k_formq, dim3(batch), dim3(256), 2 * n * sizeof(cd), s, n, Ac,
+                         (const cd*)ws.tau, ws.Z);
+    } else {
+      hipLaunchKernelGGL(k_extract_v, dim3(64, batch), dim3(256), 0, s, n, Ac, ws.X);
+      hipLaunchKernelGGL(k_larft, dim3(nblk, batch), dim3(256), 0, s, n, (const cd* const*)ws.X,
+                         (const cd*)ws.tau, W);
+      set_identity_batched(s, n, ws.Z, batch);
+      const long long offW1 = (long long)nblk * QB * QB, offW2 = offW1 + (long long)QB * n;
+      const cd* const* Xc = (const cd* const*)ws.X;
+      const cd* const* Wc = (const cd* const*)W;
+      for (int blk = nblk - 1; blk >= 0; --blk) {
+        const int j0 = blk * QB, nbk = min(QB, nref - j0), m = n - j0 - 1;
+        const long long offV = (long long)(j0 + 1) + (long long)j0 * n;
+        const long long offZ = (long long)(j0 + 1) + (long long)(j0 + 1) * n;
+        // W1 = V^H Zt   (nbk x m)
+        zgemm_batched(s, nbk, m, m, mk(1.0, 0.0), Xc, offV, n, 1, 1, (const cd* const*)ws.Z, offZ, 1,
+                      n, 0, mk(0.0, 0.0), W, offW1, 1, QB, batch);
+        // W2 = T W1
+        zgemm_batched(s, nbk, m, nbk, mk(1.0, 0.0), Wc, (long long)blk * QB * QB, 1, QB, 0, Wc,
+                      offW1, 1, QB, 0, mk(0.0, 0.0), W, offW2, 1, QB, batch);
+        // Zt -= V W2
+        zgemm_batched(s, m, m, nbk, mk(-1.0, 0.0), Xc, offV, 1, n, 0, Wc, offW2, 1, QB, 0,
+                      mk(1.0, 0.0), ws.Z, offZ, 1, n, batch);
+      }
+    }
   } else {
     set_identity_batched(s, n, ws.Z, batch);
   }

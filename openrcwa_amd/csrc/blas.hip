@@ -44,11 +44,13 @@ __global__ __launch_bounds__(256) void k_zgemm(
       accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
 
+  // global loads run along whichever index of an operand has unit stride
+  const bool a_mfast = (rsA == 1), b_kfast = (rsB == 1);
   for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int e = tid + 256 * j;
-      const int mm = e & 63, kk = e >> 6;
+      const int mm = a_mfast ? (e & 63) : (e >> 4), kk = a_mfast ? (e >> 6) : (e & 15);
       const int gm = m0 + mm, gk = k0 + kk;
       cd v = mk(0.0, 0.0);
       if (gm < M && gk < K) v = A[gm * rsA + gk * csA];
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(256) void k_zgemm(
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int e = tid + 256 * j;
-      const int kk = e & 15, nn = e >> 4;
+      const int kk = b_kfast ? (e & 15) : (e >> 6), nn = b_kfast ? (e >> 4) : (e & 63);
       const int gn = n0 + nn, gk = k0 + kk;
       cd v = mk(0.0, 0.0);
       if (gn < N && gk < K) v = B[gk * rsB + gn * csB];

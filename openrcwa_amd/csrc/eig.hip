@@ -23,14 +23,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 static constexpr double kUlp = DBL_EPSILON;
 static constexpr double kSafmin = DBL_MIN;
 
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ inline double wave_max(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
 
 // block reduction of a double (256 threads); result valid in all threads
 __device__ inline double block_sum(double v, double* sred) {
@@ -524,11 +516,6 @@ __device__ inline cd wilkinson_lds(const cd* B, int hi) {
 // wave 0 only: every lane computes the (uniform) deflation point, shift and
 // rotation redundantly, lane j owns column j / row j of the updates, and LDS
 // ordering inside the wave is enforced with wavefront-scope fences.
-__device__ inline void wave_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ __forceinline__ bool wave_small_qr(cd* B, cd* U, int m, bool wantu, int maxit) {
   const int lane = threadIdx.x & 63;

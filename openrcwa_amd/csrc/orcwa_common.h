@@ -73,6 +73,22 @@ __host__ __device__ inline cd cexp(cd z) {
   return mk(e * c, e * s);
 }
 
+// wavefront (64-lane) helpers
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// orders LDS accesses between the lanes of one wavefront
+__device__ inline void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 #define ORCWA_HIP_CHECK(expr)                                                       \
   do {                                                                              \
     hipError_t _e = (expr);                                                         \

@@ -63,7 +63,9 @@ struct ChunkTables {
   size_t pE11 = 0, pE22 = 0;     // grid convolution matrices of each problem
   size_t uV = 0;                 // homogeneous V slots
   // per instance RTCM slots
-  size_t R = 0, T = 0, E = 0, H = 0, F = 0, G = 0, LU = 0, Tau = 0;
+  size_t Rt[2] = {0, 0};          // R^T of the cascade, alternating by step parity
+  size_t E = 0, H = 0;           // interface matrices, then A = E + H, B = E - H
+  std::vector<size_t> LU;        // per step: LU of (F + G)^T
   long long prob_base = 0;       // global index of the chunk's first problem
   std::vector<StepTables> steps;
 };
@@ -115,7 +117,9 @@ struct orcwa_plan {
   cd* d_keffu = nullptr;           // [umax*n]
   cd* d_binv = nullptr;            // [umax*4h]
   cd* d_tau = nullptr;             // [pmax*n]
-  int* d_ipiv = nullptr;           // [chunk*n]
+  int* d_ipiv = nullptr;           // [max(chunk, pmax)*n] scratch pivots
+  int* d_spiv = nullptr;           // [L-1][chunk*n] pivots of each cascade step
+  cd* d_rtv = nullptr;             // [chunk][4][n] Rud / Tdd columns i0, h+i0
   int* d_pinfo = nullptr;          // [nprob_total]
   double* d_resid = nullptr;       // [nprob_total]
   // outputs
@@ -332,7 +336,8 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   }
   double budget_gb = 32.0;
   if (const char* s = getenv("ORCWA_WORKSPACE_GB")) budget_gb = atof(s);
-  const size_t per_inst = (size_t)(7 * lp_max + lu_max + 8) * slot;
+  const int islots = L + 3;      // Rt x2, E, H, one LU per cascade step
+  const size_t per_inst = (size_t)(7 * lp_max + lu_max + islots) * slot;
   long long c_fit = (long long)(budget_gb * 1e9 / (double)per_inst);
   int chunk = (int)std::max(1LL, std::min((long long)count, c_fit));
   if (const char* s = getenv("ORCWA_CHUNK")) chunk = std::max(1, std::min(count, atoi(s)));
@@ -389,7 +394,9 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   const int PM = std::max(1, p->pmax), UM = std::max(1, p->umax);
   const size_t o_pslots = take((size_t)PM * 7 * slot);
   const size_t o_uslots = take((size_t)UM * slot);
-  const size_t o_islots = take((size_t)chunk * 8 * slot);
+  const size_t o_islots = take((size_t)chunk * islots * slot);
+  const size_t o_spiv = take((size_t)(L - 1) * chunk * n * sizeof(int));
+  const size_t o_rtv = take((size_t)chunk * 4 * n * sizeof(cd));
   const size_t o_lam = take((size_t)PM * n * sizeof(cd));
   const size_t o_keffp = take((size_t)PM * n * sizeof(cd));
   const size_t o_keffu = take((size_t)UM * n * sizeof(cd));
@@ -432,6 +439,8 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   p->d_binv = (cd*)(W + o_binv);
   p->d_tau = (cd*)(W + o_tau);
   p->d_ipiv = (int*)(W + o_ipiv);
+  p->d_spiv = (int*)(W + o_spiv);
+  p->d_rtv = (cd*)(W + o_rtv);
   p->d_pinfo = (int*)(W + o_pinfo);
   p->d_resid = (double*)(W + o_resid);
   p->d_ref = (double*)(W + o_ref);
@@ -486,7 +495,9 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
 
   auto pslot = [&](int pi, int k) { return (cd*)(W + o_pslots + ((size_t)pi * 7 + k) * slot); };
   auto uslot = [&](int ui) { return (cd*)(W + o_uslots + (size_t)ui * slot); };
-  auto islot = [&](int ii, int k) { return (cd*)(W + o_islots + ((size_t)ii * 8 + k) * slot); };
+  auto islot = [&](int ii, int k) {
+    return (cd*)(W + o_islots + ((size_t)ii * islots + k) * slot);
+  };
 
   long long prob_global = 0;
   for (int c = 0; c < p->nchunks; ++c) {
@@ -561,25 +572,24 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
       ct.pE11 = B.put(e11);
       ct.pE22 = B.put(e22);
     }
-    std::vector<cd*> tR, tT, tE, tH, tF, tG, tLU, tTau;
-    for (int ii = 0; ii < ct.ninst; ++ii) {
-      tR.push_back(islot(ii, 0));
-      tT.push_back(islot(ii, 1));
-      tE.push_back(islot(ii, 2));
-      tH.push_back(islot(ii, 3));
-      tF.push_back(islot(ii, 4));
-      tG.push_back(islot(ii, 5));
-      tLU.push_back(islot(ii, 6));
-      tTau.push_back(islot(ii, 7));
+    {
+      std::vector<cd*> tR0, tR1, tE, tH;
+      for (int ii = 0; ii < ct.ninst; ++ii) {
+        tR0.push_back(islot(ii, 0));
+        tR1.push_back(islot(ii, 1));
+        tE.push_back(islot(ii, 2));
+        tH.push_back(islot(ii, 3));
+      }
+      ct.Rt[0] = B.put(tR0);
+      ct.Rt[1] = B.put(tR1);
+      ct.E = B.put(tE);
+      ct.H = B.put(tH);
+      for (int si = 0; si < L - 1; ++si) {
+        std::vector<cd*> tLU;
+        for (int ii = 0; ii < ct.ninst; ++ii) tLU.push_back(islot(ii, 4 + si));
+        ct.LU.push_back(B.put(tLU));
+      }
     }
-    ct.R = B.put(tR);
-    ct.T = B.put(tT);
-    ct.E = B.put(tE);
-    ct.H = B.put(tH);
-    ct.F = B.put(tF);
-    ct.G = B.put(tG);
-    ct.LU = B.put(tLU);
-    ct.Tau = B.put(tTau);
     // layer references
     auto Wof = [&](int ii, int l) -> cd* {
       int q = pidx[(size_t)ii * L + l];
@@ -623,7 +633,7 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
           paVb.push_back(Vof(ii, b));
           paE.push_back(islot(ii, 2));
           paH.push_back(islot(ii, 3));
-          paLU.push_back(islot(ii, 6));
+          paLU.push_back(islot(ii, 4 + sidx));
           paInst.push_back(i);
         }
       }
@@ -770,20 +780,16 @@ int orcwa_plan_execute(orcwa_plan* p) {
     HIPCHK(hipEventRecord(p->ev[3], s));
 
     // ---- 2d-e. interfaces and the RTCM cascade, exit side first
-    cd* const* R = p->dev<cd* const*>(ct.R);
-    cd* const* T = p->dev<cd* const*>(ct.T);
     cd* const* E = p->dev<cd* const*>(ct.E);
     cd* const* H = p->dev<cd* const*>(ct.H);
-    cd* const* F = p->dev<cd* const*>(ct.F);
-    cd* const* Gm = p->dev<cd* const*>(ct.G);
-    cd* const* LU = p->dev<cd* const*>(ct.LU);
-    cd* const* Tau = p->dev<cd* const*>(ct.Tau);
     auto C = [](cd* const* x) { return (const cd* const*)x; };
-    launch_zero_mats(s, nn, R, ni);
-    set_identity_batched(s, n, T, ni);
+    const int i0 = g.ny * (g.nx - 1) / 2 + (g.ny - 1) / 2;
     for (int si = 0; si < p->L - 1; ++si) {
       const StepTables& st = ct.steps[si];
-      // interfaces
+      const bool last = (si == p->L - 2);
+      cd* const* M = p->dev<cd* const*>(ct.LU[si]);
+      int* piv = p->d_spiv + (size_t)si * p->chunk * n;
+      // interfaces E = W_a^-1 W_b, H = V_a^-1 V_b (rcwa/RCWASolver.cpp:495-569)
       if (st.ua.count > 0) {
         const StepGroup& q = st.ua;
         copy_batched(s, nn, p->dev<const cd* const*>(q.Wb), p->dev<cd* const*>(q.E), q.count);
@@ -802,37 +808,48 @@ int orcwa_plan_execute(orcwa_plan* p) {
         zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
         zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.H), q.count);
       }
-      // cascade step
-      if (si > 0) {
-        launch_phase_scale(s, n, p->dev<const cd* const*>(st.keff_b), p->dev<const double*>(st.k0d),
-                           R, T, ni);
-        // F = E + E R~, G = H - H R~
-        copy_batched(s, nn, C(E), F, ni);
-        zgemm_batched(s, n, n, n, one, C(E), 0, 1, n, 0, C(R), 0, 1, n, 0, one, F, 0, 1, n, ni);
-        copy_batched(s, nn, C(H), Gm, ni);
-        zgemm_batched(s, n, n, n, mk(-1.0, 0.0), C(H), 0, 1, n, 0, C(R), 0, 1, n, 0, one, Gm, 0, 1,
-                      n, ni);
-      } else {
-        copy_batched(s, nn, C(E), F, ni);
-        copy_batched(s, nn, C(H), Gm, ni);
+      // A = E + H (in E), B = E - H (in H)
+      launch_sum_diff(s, n, E, H, ni);
+      // R~ = phi R phi, kept transposed (the scaling is symmetric in i, j)
+      cd* const* Rcur = p->dev<cd* const*>(ct.Rt[si & 1]);
+      cd* const* Rnext = p->dev<cd* const*>(ct.Rt[(si + 1) & 1]);
+      if (si > 0)
+        launch_phase_scale(s, n, p->dev<const cd* const*>(st.keff_b),
+                           p->dev<const double*>(st.k0d), Rcur, nullptr, ni);
+      // M^T = (F + G)^T = A^T + (B R~)^T; the GEMM reads R~ through R~^T and
+      // stores its product transposed
+      launch_transpose(s, n, C(E), M, ni);
+      if (si > 0)
+        zgemm_batched(s, n, n, n, one, C(H), 0, 1, n, 0, C(Rcur), 0, n, 1, 0, one, M, 0, n, 1,
+                      ni);
+      if (!last) {
+        // (F - G)^T = B^T + (A R~)^T
+        launch_transpose(s, n, C(H), Rnext, ni);
+        if (si > 0)
+          zgemm_batched(s, n, n, n, one, C(E), 0, 1, n, 0, C(Rcur), 0, n, 1, 0, one, Rnext, 0, n,
+                        1, ni);
       }
-      // tau = (F + G)^-1
-      axpby_batched(s, n, one, C(F), one, C(Gm), LU, ni);
-      zgetrf_batched(s, n, LU, p->d_ipiv, p->d_info + ct.i0, nullptr, ni);
-      set_identity_batched(s, n, Tau, ni);
-      zgetrs_batched(s, n, n, C(LU), p->d_ipiv, Tau, ni);
-      // R = I - 2 G tau ; T = 2 T~ tau
-      set_identity_batched(s, n, R, ni);
-      zgemm_batched(s, n, n, n, mk(-2.0, 0.0), C(Gm), 0, 1, n, 0, C(Tau), 0, 1, n, 0, one, R, 0, 1,
-                    n, ni);
-      zgemm_batched(s, n, n, n, mk(2.0, 0.0), C(T), 0, 1, n, 0, C(Tau), 0, 1, n, 0, zero, E, 0, 1,
-                    n, ni);
-      copy_batched(s, nn, C(E), T, ni);
+      zgetrf_batched(s, n, M, piv, p->d_info + ct.i0, nullptr, ni);
+      // R_new^T = (F + G)^-T (F - G)^T
+      if (!last) zgetrs_batched(s, n, n, C(M), piv, Rnext, ni);
+    }
+    {
+      // columns i0, h+i0 of Rud and Tdd
+      const int sl = p->L - 2;
+      launch_cascade_tail(s, n, i0, p->dev<const cd* const*>(ct.LU[sl]),
+                          p->d_spiv + (size_t)sl * p->chunk * n, C(E), C(H),
+                          sl > 0 ? p->dev<const cd* const*>(ct.Rt[sl & 1]) : nullptr, p->d_rtv,
+                          ni);
+      for (int sc = sl - 1; sc >= 0; --sc)
+        launch_cascade_chain(s, n, p->dev<const cd* const*>(ct.LU[sc]),
+                             p->d_spiv + (size_t)sc * p->chunk * n,
+                             p->dev<const cd* const*>(ct.steps[sc + 1].keff_b),
+                             p->dev<const double*>(ct.steps[sc + 1].k0d), p->d_rtv, ni);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(p->ev[4], s));
     // ---- 2f. plane-wave response
-    launch_rt(s, g, p->dev<const RtParam*>(ct.rt_params), C(R), C(T),
+    launch_rt(s, g, p->dev<const RtParam*>(ct.rt_params), p->d_rtv,
               p->d_ref + (size_t)ct.i0 * p->maxpol * h, p->d_trn + (size_t)ct.i0 * p->maxpol * h,
               p->d_amp + (size_t)ct.i0 * 8 * h, p->d_info + ct.i0, p->maxpol, ni);
     HIPCHK(hipGetLastError());

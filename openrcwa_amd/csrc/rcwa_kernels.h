@@ -79,8 +79,14 @@ void launch_bdinv_apply(hipStream_t s, int h, const cd* const* binv, const cd* c
                         cd* const* H, int batch);
 void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const double* k0d,
                         cd* const* R, cd* const* T, int batch);
-void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* const* R,
-               const cd* const* T, double* ref, double* trn, cd* amp, int* info, int max_pol,
-               int batch);
+void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* rt,
+               double* ref, double* trn, cd* amp, int* info, int max_pol, int batch);
+void launch_transpose(hipStream_t s, int n, const cd* const* src, cd* const* dst, int batch);
+void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, int batch);
+void launch_cascade_tail(hipStream_t s, int n, int i0, const cd* const* LUt, const int* piv,
+                         const cd* const* A, const cd* const* B, const cd* const* Rt, cd* rt,
+                         int batch);
+void launch_cascade_chain(hipStream_t s, int n, const cd* const* LUt, const int* piv,
+                          const cd* const* keff, const double* k0d, cd* rt, int batch);
 
 }  // namespace orcwa

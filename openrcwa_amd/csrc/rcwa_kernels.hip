@@ -7,7 +7,8 @@
 //   k_build_pq   : P*Q of rcwa/Layer.cpp:150-220 + rcwa/WaveEquationSolver.cpp:30
 //   k_build_v    : V = Q W diag(1/keff), rcwa/WaveEquationSolver.cpp:33-36
 //   k_uniform    : closed-form modes of a homogeneous layer
-//   k_phase_scale, k_bdinv_apply : pieces of the RTCM cascade and interfaces
+//   k_phase_scale, k_bdinv_apply, k_transpose, k_sum_diff, k_cascade_tail,
+//   k_cascade_chain : pieces of the RTCM cascade and interfaces
 //       rcwa/RCWASolver.cpp:378-614
 //   k_rt         : plane-wave response and per-order efficiencies
 //       rcwa/RCWASolver.cpp:1778-1908
@@ -370,24 +371,220 @@ __global__ void k_phase_scale(int n, const cd* const* keff, const double* k0d, c
     const cd pr = cexp(mk(-ar.im, ar.re));
     const cd pc = cexp(mk(-ac.im, ac.re));
     R[b][e] = pr * R[b][e] * pc;
-    T[b][e] = T[b][e] * pc;
+    if (T) T[b][e] = T[b][e] * pc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RTCM cascade pieces (rcwa/RCWASolver.cpp:378-492), rewritten so that no
+// explicit inverse is formed:
+//   with A = E + H, B = E - H and R~ = phi R phi,
+//   F + G = A + B R~,  F - G = B + A R~,
+//   R_new = I - 2 G tau = (F - G)(F + G)^-1,      tau = (F + G)^-1,
+//   T_new = 2 (T phi) tau.
+// R_new is obtained from the transposed system (F+G)^T R_new^T = (F-G)^T, so
+// the cascade keeps R^T; its LU factors M^T = P L U (M = F + G) are kept per
+// step, and T enters only through two columns, carried right to left:
+//   Tdd e_c = 2 tau_0 (phi_1 2 tau_1 (... (phi_last 2 tau_last e_c))).
+// ---------------------------------------------------------------------------
+__global__ void k_transpose(int n, const cd* const* src, cd* const* dst) {
+  __shared__ cd t[32][33];
+  const cd* S = src[blockIdx.z];
+  cd* D = dst[blockIdx.z];
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 256 threads: 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + tx, c = c0 + k;
+    if (r < n && c < n) t[k][tx] = S[r + (long long)c * n];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int r = c0 + tx, c = r0 + k;   // D(r, c) = S(c, r)
+    if (r < n && c < n) D[r + (long long)c * n] = t[tx][k];
+  }
+}
+
+__global__ void k_sum_diff(long long count, cd* const* Ep, cd* const* Hp) {
+  cd* E = Ep[blockIdx.y];
+  cd* H = Hp[blockIdx.y];
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x) {
+    const cd a = E[e], c = H[e];
+    E[e] = a + c;
+    H[e] = a - c;
+  }
+}
+
+// x <- M^-1 x for the two vectors x0, x1 (LDS), given zgetrf of M^T:
+// M^T = P L U  =>  M = U^T L^T P^T  =>  x = P L^-T U^-T x.  Wave 0 only.
+__device__ void lu_t_solve2(int n, const cd* __restrict__ LU, const int* __restrict__ piv,
+                            cd* x0, cd* x1) {
+  const int lane = threadIdx.x & 63;
+  // U^T y = x, forward: y_i = (x_i - sum_{j<i} U(j,i) y_j) / U(i,i); U(j,i) = column i
+  for (int i = 0; i < n; ++i) {
+    const cd* col = LU + (long long)i * n;
+    double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+    for (int j = lane; j < i; j += 64) {
+      const cd u = col[j], y0 = x0[j], y1 = x1[j];
+      a0r += u.re * y0.re - u.im * y0.im;
+      a0i += u.re * y0.im + u.im * y0.re;
+      a1r += u.re * y1.re - u.im * y1.im;
+      a1i += u.re * y1.im + u.im * y1.re;
+    }
+    a0r = wave_sum(a0r);
+    a0i = wave_sum(a0i);
+    a1r = wave_sum(a1r);
+    a1i = wave_sum(a1i);
+    if (lane == 0) {
+      const cd d = cinv(col[i]);
+      x0[i] = (x0[i] - mk(a0r, a0i)) * d;
+      x1[i] = (x1[i] - mk(a1r, a1i)) * d;
+    }
+    wave_fence();
+  }
+  // L^T z = y, backward, unit diagonal: z_i = y_i - sum_{j>i} L(j,i) z_j
+  for (int i = n - 2; i >= 0; --i) {
+    const cd* col = LU + (long long)i * n;
+    double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+    for (int j = i + 1 + lane; j < n; j += 64) {
+      const cd l = col[j], z0 = x0[j], z1 = x1[j];
+      a0r += l.re * z0.re - l.im * z0.im;
+      a0i += l.re * z0.im + l.im * z0.re;
+      a1r += l.re * z1.re - l.im * z1.im;
+      a1i += l.re * z1.im + l.im * z1.re;
+    }
+    a0r = wave_sum(a0r);
+    a0i = wave_sum(a0i);
+    a1r = wave_sum(a1r);
+    a1i = wave_sum(a1i);
+    if (lane == 0) {
+      x0[i] -= mk(a0r, a0i);
+      x1[i] -= mk(a1r, a1i);
+    }
+    wave_fence();
+  }
+  // x = P z: the interchanges of zgetrf in reverse order
+  if (lane == 0) {
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = piv[k];
+      if (p != k) {
+        cd t = x0[k]; x0[k] = x0[p]; x0[p] = t;
+        t = x1[k]; x1[k] = x1[p]; x1[p] = t;
+      }
+    }
+  }
+  wave_fence();
+}
+
+// Last cascade step (the interface at the incident half-space): for
+// c in {i0, h+i0}: x = (F+G)^-1 e_c, Rud e_c = (F-G) x = B x + A (R~ x),
+// 2 x starts the Tdd column chain.  Rt = R~^T of the previous steps, or null
+// for a two-layer stack.  One 256-thread workgroup per instance, LDS 4n cd.
+__global__ __launch_bounds__(256) void k_cascade_tail(int n, int i0, const cd* const* LUt,
+                                                      const int* piv_all, const cd* const* Ap,
+                                                      const cd* const* Bp, const cd* const* Rtp,
+                                                      cd* rt_all) {
+  extern __shared__ cd sm[];
+  cd* x0 = sm;
+  cd* x1 = sm + n;
+  cd* u0 = sm + 2 * n;
+  cd* u1 = sm + 3 * n;
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int h = n / 2;
+  for (int i = tid; i < n; i += 256) {
+    x0[i] = mk(i == i0 ? 1.0 : 0.0, 0.0);
+    x1[i] = mk(i == h + i0 ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  if (wave == 0) lu_t_solve2(n, LUt[b], piv_all + (long long)b * n, x0, x1);
+  __syncthreads();
+  // u = R~ x, R~(i, j) = Rt(j, i): column i of Rt
+  const cd* Rt = Rtp ? Rtp[b] : nullptr;
+  for (int i = wave; i < n; i += 4) {
+    double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+    if (Rt) {
+      const cd* col = Rt + (long long)i * n;
+      for (int j = lane; j < n; j += 64) {
+        const cd r = col[j], y0 = x0[j], y1 = x1[j];
+        a0r += r.re * y0.re - r.im * y0.im;
+        a0i += r.re * y0.im + r.im * y0.re;
+        a1r += r.re * y1.re - r.im * y1.im;
+        a1i += r.re * y1.im + r.im * y1.re;
+      }
+      a0r = wave_sum(a0r);
+      a0i = wave_sum(a0i);
+      a1r = wave_sum(a1r);
+      a1i = wave_sum(a1i);
+    }
+    if (lane == 0) {
+      u0[i] = mk(a0r, a0i);
+      u1[i] = mk(a1r, a1i);
+    }
+  }
+  __syncthreads();
+  const cd* A = Ap[b];
+  const cd* B = Bp[b];
+  cd* out = rt_all + (long long)b * 4 * n;
+  for (int i = tid; i < n; i += 256) {
+    cd r0 = mk(0.0, 0.0), r1 = mk(0.0, 0.0);
+    for (int j = 0; j < n; ++j) {
+      const cd bij = B[i + (long long)j * n], aij = A[i + (long long)j * n];
+      r0 += bij * x0[j] + aij * u0[j];
+      r1 += bij * x1[j] + aij * u1[j];
+    }
+    out[i] = r0;
+    out[n + i] = r1;
+    out[2 * n + i] = 2.0 * x0[i];
+    out[3 * n + i] = 2.0 * x1[i];
+  }
+}
+
+// One link of the Tdd column chain: t <- 2 (F+G)_s^-1 (phi_{s+1} t), with
+// phi = exp(i keff (k0 d)) of the layer propagated at step s+1.
+__global__ __launch_bounds__(256) void k_cascade_chain(int n, const cd* const* LUt,
+                                                       const int* piv_all, const cd* const* keff,
+                                                       const double* k0d, cd* rt_all) {
+  extern __shared__ cd sm[];
+  cd* x0 = sm;
+  cd* x1 = sm + n;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  cd* t = rt_all + (long long)b * 4 * n + 2 * n;
+  const cd* kf = keff[b];
+  const double kd = k0d[b];
+  for (int i = tid; i < n; i += 256) {
+    const cd a = kd * kf[i];
+    const cd ph = cexp(mk(-a.im, a.re));
+    x0[i] = t[i] * ph;
+    x1[i] = t[n + i] * ph;
+  }
+  __syncthreads();
+  if ((tid >> 6) == 0) lu_t_solve2(n, LUt[b], piv_all + (long long)b * n, x0, x1);
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    t[i] = 2.0 * x0[i];
+    t[n + i] = 2.0 * x1[i];
   }
 }
 
 // Plane-wave response and efficiencies (rcwa/RCWASolver.cpp:1778-1908) for a
 // stack whose first and last layers are homogeneous (W = I there): the
 // incident coefficient vector is s = [px delta; py delta], r = Rud s,
-// t = Tdd s.  One workgroup per instance.
+// t = Tdd s, so only columns i0 and h+i0 of Rud and Tdd enter; they arrive as
+// [4][n] per instance (Rud col i0, Rud col h+i0, Tdd col i0, Tdd col h+i0).
+// One workgroup per instance.
 __global__ __launch_bounds__(256) void k_rt(const ConvGeom g, const RtParam* params,
-                                            const cd* const* Rud, const cd* const* Tdd,
-                                            double* ref_out, double* trn_out, cd* amp_out,
-                                            int* info, int max_pol) {
+                                            const cd* rt_all, double* ref_out,
+                                            double* trn_out, cd* amp_out, int* info,
+                                            int max_pol) {
   const int b = blockIdx.x;
   const RtParam p = params[b];
   const int h = g.nx * g.ny, n = 2 * h;
   const int i0 = g.ny * (g.nx - 1) / 2 + (g.ny - 1) / 2;
-  const cd* R = Rud[b];
-  const cd* T = Tdd[b];
+  // columns i0 and h+i0 of Rud and Tdd
+  const cd* R0 = rt_all + (long long)b * 4 * n;
+  const cd* R1 = R0 + n;
+  const cd* T0 = R0 + 2 * n;
+  const cd* T1 = R0 + 3 * n;
   cd* amp = amp_out + (long long)b * 8 * h;
   __shared__ int s_graze;
   if (threadIdx.x == 0) s_graze = 0;
@@ -403,14 +600,14 @@ __global__ __launch_bounds__(256) void k_rt(const ConvGeom g, const RtParam* par
   }
   for (int k = threadIdx.x; k < h; k += blockDim.x) {
     // basis responses: column i0 (x-polarised) and h+i0 (y-polarised)
-    amp[0 * h + k] = R[k + (long long)i0 * n];
-    amp[1 * h + k] = R[h + k + (long long)i0 * n];
-    amp[2 * h + k] = T[k + (long long)i0 * n];
-    amp[3 * h + k] = T[h + k + (long long)i0 * n];
-    amp[4 * h + k] = R[k + (long long)(h + i0) * n];
-    amp[5 * h + k] = R[h + k + (long long)(h + i0) * n];
-    amp[6 * h + k] = T[k + (long long)(h + i0) * n];
-    amp[7 * h + k] = T[h + k + (long long)(h + i0) * n];
+    amp[0 * h + k] = R0[k];
+    amp[1 * h + k] = R0[h + k];
+    amp[2 * h + k] = T0[k];
+    amp[3 * h + k] = T0[h + k];
+    amp[4 * h + k] = R1[k];
+    amp[5 * h + k] = R1[h + k];
+    amp[6 * h + k] = T1[k];
+    amp[7 * h + k] = T1[h + k];
     double kx, ky;
     k_vectors(g, pp, k, kx, ky);
     const cd kzr = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
@@ -431,10 +628,10 @@ __global__ __launch_bounds__(256) void k_rt(const ConvGeom g, const RtParam* par
         to[k] = 0.0;
         continue;
       }
-      const cd rx = px * R[k + (long long)i0 * n] + py * R[k + (long long)(h + i0) * n];
-      const cd ry = px * R[h + k + (long long)i0 * n] + py * R[h + k + (long long)(h + i0) * n];
-      const cd tx = px * T[k + (long long)i0 * n] + py * T[k + (long long)(h + i0) * n];
-      const cd ty = px * T[h + k + (long long)i0 * n] + py * T[h + k + (long long)(h + i0) * n];
+      const cd rx = px * R0[k] + py * R1[k];
+      const cd ry = px * R0[h + k] + py * R1[h + k];
+      const cd tx = px * T0[k] + py * T1[k];
+      const cd ty = px * T0[h + k] + py * T1[h + k];
       double kx, ky;
       k_vectors(g, pp, k, kx, ky);
       const cd kzr = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
@@ -528,12 +725,39 @@ void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const doubl
                      n, keff, k0d, R, T);
 }
 
-void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* const* R,
-               const cd* const* T, double* ref, double* trn, cd* amp, int* info, int max_pol,
-               int batch) {
+void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* rt,
+               double* ref, double* trn, cd* amp, int* info, int max_pol, int batch) {
   if (batch <= 0) return;
-  hipLaunchKernelGGL(k_rt, dim3(batch), dim3(256), 0, s, g, params, R, T, ref, trn, amp, info,
+  hipLaunchKernelGGL(k_rt, dim3(batch), dim3(256), 0, s, g, params, rt, ref, trn, amp, info,
                      max_pol);
+}
+
+void launch_transpose(hipStream_t s, int n, const cd* const* src, cd* const* dst, int batch) {
+  if (batch <= 0) return;
+  const unsigned t = (unsigned)((n + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, dim3(t, t, batch), dim3(256), 0, s, n, src, dst);
+}
+
+void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, int batch) {
+  if (batch <= 0) return;
+  long long b = blocks_for((long long)n * n);
+  hipLaunchKernelGGL(k_sum_diff, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s,
+                     (long long)n * n, E, H);
+}
+
+void launch_cascade_tail(hipStream_t s, int n, int i0, const cd* const* LUt, const int* piv,
+                         const cd* const* A, const cd* const* B, const cd* const* Rt, cd* rt,
+                         int batch) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_cascade_tail, dim3(batch), dim3(256), 4 * n * sizeof(cd), s, n, i0, LUt,
+                     piv, A, B, Rt, rt);
+}
+
+void launch_cascade_chain(hipStream_t s, int n, const cd* const* LUt, const int* piv,
+                          const cd* const* keff, const double* k0d, cd* rt, int batch) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_cascade_chain, dim3(batch), dim3(256), 2 * n * sizeof(cd), s, n, LUt, piv,
+                     keff, k0d, rt);
 }
 
 }  // namespace orcwa

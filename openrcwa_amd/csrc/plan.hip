@@ -334,12 +334,22 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
     lp_max = std::max(lp_max, a);
     lu_max = std::max(lu_max, b);
   }
-  double budget_gb = 32.0;
+  // workspace budget: ORCWA_WORKSPACE_GB, else 70% of the device memory free
+  // now (the per-instance slots dominate; a few large chunks keep every
+  // batched kernel wide and pay the end-of-launch tail few times)
+  double budget_gb = 0.0;
   if (const char* s = getenv("ORCWA_WORKSPACE_GB")) budget_gb = atof(s);
+  if (!(budget_gb > 0.0)) {
+    size_t fr = 0, tot = 0;
+    budget_gb = (hipMemGetInfo(&fr, &tot) == hipSuccess) ? 0.7 * (double)fr / 1e9 : 8.0;
+  }
   const int islots = L + 3;      // Rt x2, E, H, one LU per cascade step
   const size_t per_inst = (size_t)(7 * lp_max + lu_max + islots) * slot;
   long long c_fit = (long long)(budget_gb * 1e9 / (double)per_inst);
-  int chunk = (int)std::max(1LL, std::min((long long)count, c_fit));
+  c_fit = std::max(1LL, std::min((long long)count, c_fit));
+  // equal chunks
+  const long long nck = (count + c_fit - 1) / c_fit;
+  int chunk = (int)((count + nck - 1) / nck);
   if (const char* s = getenv("ORCWA_CHUNK")) chunk = std::max(1, std::min(count, atoi(s)));
   p->chunk = chunk;
   p->nchunks = (count + chunk - 1) / chunk;

@@ -31,7 +31,7 @@ and ``PartialPivLU``.  Here ``numpy.linalg.eig`` is LAPACK ``zgeev`` (OpenBLAS
 complete-orthogonal-factorisation least-squares driver) and BDCSVD solves are
 LAPACK ``zgelsd`` (divide-and-conquer SVD least squares), via scipy.
 
-Pinning: every function is checked in ``tests/test_oracle_*.py`` against the
+Pinning: every function is checked in ``tests/test_oracle.py`` against the
 reference's published values (AGENTS.md:94-95, 114-120), closed forms
 (Fresnel, Brewster, Airy film, uniform-layer eigenvalues, exact Fourier
 integrals by quadrature) and the invariances the reference's CI asserts.  Two

@@ -43,3 +43,13 @@ def test_errors_without_device_are_reported():
     h = ctypes.c_void_p()
     rc = lib.orcwa_gpu_create(ctypes.byref(h), 999)
     assert rc != 0
+
+
+def test_rcwa_run_rejects_an_invalid_setup():
+    # fewer than two layers: "*** invalid rcwa setup" and return 1 before any
+    # device work (the reference's bridge returns 1 on errors, sol/rcwa_bridge.cpp:145)
+    from openrcwa_amd import _lib
+    lib = _lib.load()
+    ctypes.c_int.in_dll(lib, "NRcwaLayer").value = 1
+    lib.rcwa_run.argtypes = [ctypes.c_void_p]
+    assert lib.rcwa_run(None) == 1

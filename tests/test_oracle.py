@@ -348,3 +348,38 @@ def test_slab_scatter_matrix_uniform_closed_form():
     st = o.redheffer_star(st, R1, T1)
     st = o.redheffer_star(st, R1, T1)
     assert np.abs(st[1] - np.diag(X)).max() < 1e-12
+
+
+def test_redheffer_star_of_patterned_slabs_equals_rtcm():
+    # slabs in the incidence medium's basis, composed with the star product
+    # (core/RCWAIntegrator.cpp:57-67), reproduce the RTCM cascade
+    # (rcwa/RCWASolver.cpp:378-492) of the same stack: reflection terms of
+    # every star enter through the later ones
+    lam = 5.5e-7
+    xe, ye = [0.0, 1.2e-7, 2.6e-7, 4.0e-7], [0.0, 1.8e-7, 4.0e-7]
+
+    def grid(e1, e2):
+        g = np.full((2, 3), e2, dtype=complex)
+        g[0, :2] = e1
+        return g
+
+    air = o.Layer(xe, ye, np.full((2, 3), 1.0 + 0j))
+    slabs = [o.Layer(xe, ye, grid(6.0 + 0.1j, 1.0)), o.Layer(xe, ye, grid(2.25, 4.0 + 0.05j)),
+             o.Layer(xe, ye, grid(1.0, 9.0))]
+    d = [1.3e-7, 0.9e-7, 0.6e-7]
+    s = o.RCWASolver(2, 2)
+    s.disable_pml()
+    for lay in [air] + slabs:
+        s.add_layer(lay)
+    k0 = 2 * np.pi / lam
+    s.set_bloch_wavevector(0.3 * k0, 0.1 * k0)
+    s.solve(lam, [0, 1, 2, 3, 0], [0.0] + d + [0.0])
+    n = s.Rud.shape[0]
+    eye, zero = np.eye(n, dtype=complex), np.zeros((n, n), complex)
+    st = (zero, eye, zero, eye)
+    for lay, dz in zip(slabs, d):
+        R, T = o.slab_scatter_matrix(lay.W, lay.V, lay.gamma * k0, air.W, air.V, dz, lam)
+        st = o.redheffer_star(st, R, T)
+    # (a dropped or transposed term is an O(1) difference)
+    assert np.abs(st[0] - s.Rud).max() < 1e-10
+    assert np.abs(st[3] - s.Tdd).max() < 1e-10

@@ -2,6 +2,9 @@
 // (v_mfma_f64_16x16x4f64), blocked LU with partial pivoting and the two
 // triangular solves.  These carry the mode-matching solves and the cascade
 // products of the layer-stack solve (rcwa/RCWASolver.cpp:378-614).
+#include <algorithm>
+#include <cstdlib>
+
 #include "engine.h"
 
 namespace orcwa {
@@ -15,7 +18,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // MFMA f64 16x16x4 lane layout (measured on gfx950): A[i][k] in lane i+16k,
 // B[k][j] in lane j+16k, D[i][j] in lane j+16(i%4), element i/4.
 // ---------------------------------------------------------------------------
-template <bool BETA0>
+// GAUSS3: the 3-multiplication complex product (P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2): 3 MFMAs per
+// complex update instead of 4, with a normwise error of the same order.
+template <bool BETA0, bool GAUSS3>
 __global__ __launch_bounds__(256) void k_zgemm(
     int M, int N, int K, cd alpha, cd beta,
     const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
@@ -35,39 +41,62 @@ __global__ __launch_bounds__(256) void k_zgemm(
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
 
-  v4d accR[2][2], accI[2][2];
+  // global loads run along whichever index of an operand has unit stride;
+  // each thread moves 4 elements of A and 4 of B per K step of 16
+  const bool a_mfast = (rsA == 1), b_kfast = (rsB == 1);
+  int amm[4], akk[4], bkk[4], bnn[4];
+  const cd* pa[4];
+  const cd* pb[4];
+  bool aok[4], bok[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    amm[j] = a_mfast ? (e & 63) : (e >> 4);
+    akk[j] = a_mfast ? (e >> 6) : (e & 15);
+    bkk[j] = b_kfast ? (e & 15) : (e >> 6);
+    bnn[j] = b_kfast ? (e >> 4) : (e & 63);
+    aok[j] = (m0 + amm[j] < M);
+    bok[j] = (n0 + bnn[j] < N);
+    pa[j] = A + (long long)(m0 + amm[j]) * rsA + (long long)akk[j] * csA;
+    pb[j] = B + (long long)bkk[j] * rsB + (long long)(n0 + bnn[j]) * csB;
+  }
+  const long long stepA = 16 * csA, stepB = 16 * rsB;
+  cd ra[4], rb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ra[j] = (aok[j] && akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
+    rb[j] = (bok[j] && bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
+  }
+
+  v4d accR[2][2], accI[2][2], accS[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
       accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      accS[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
 
-  // global loads run along whichever index of an operand has unit stride
-  const bool a_mfast = (rsA == 1), b_kfast = (rsB == 1);
   for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      const int mm = a_mfast ? (e & 63) : (e >> 4), kk = a_mfast ? (e >> 6) : (e & 15);
-      const int gm = m0 + mm, gk = k0 + kk;
-      cd v = mk(0.0, 0.0);
-      if (gm < M && gk < K) v = A[gm * rsA + gk * csA];
-      sAr[kk][mm] = v.re;
-      sAi[kk][mm] = sa * v.im;
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      const int kk = b_kfast ? (e & 15) : (e >> 6), nn = b_kfast ? (e >> 4) : (e & 63);
-      const int gn = n0 + nn, gk = k0 + kk;
-      cd v = mk(0.0, 0.0);
-      if (gn < N && gk < K) v = B[gk * rsB + gn * csB];
-      sBr[kk][nn] = v.re;
-      sBi[kk][nn] = sb * v.im;
+      sAr[akk[j]][amm[j]] = ra[j].re;
+      sAi[akk[j]][amm[j]] = sa * ra[j].im;
+      sBr[bkk[j]][bnn[j]] = rb[j].re;
+      sBi[bkk[j]][bnn[j]] = sb * rb[j].im;
     }
     __syncthreads();
+    // next K step in flight while this one runs on the MFMA units
+    if (k0 + 16 < K) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        pa[j] += stepA;
+        pb[j] += stepB;
+        ra[j] = (aok[j] && k0 + 16 + akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
+        rb[j] = (bok[j] && k0 + 16 + bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
+      }
+    }
 #pragma unroll
     for (int kq = 0; kq < 16; kq += 4) {
       const int ka = kq + (lane >> 4);
@@ -79,15 +108,32 @@ __global__ __launch_bounds__(256) void k_zgemm(
         br[i] = sBr[ka][wn + 16 * i + (lane & 15)];
         bi[i] = sBi[ka][wn + 16 * i + (lane & 15)];
       }
+      if (GAUSS3) {
+        double as[2], bs[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+        for (int i = 0; i < 2; ++i) {
+          as[i] = ar[i] + ai[i];
+          bs[i] = br[i] + bi[i];
         }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], accI[i][j], 0, 0, 0);
+            accS[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], accS[i][j], 0, 0, 0);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+          }
+      }
     }
     __syncthreads();
   }
@@ -101,12 +147,28 @@ __global__ __launch_bounds__(256) void k_zgemm(
         const int gm = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
         const int gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N) {
-          cd v = alpha * mk(accR[i][j][r], accI[i][j][r]);
+          cd acc;
+          if (GAUSS3) {
+            const double p1 = accR[i][j][r], p2 = accI[i][j][r];
+            acc = mk(p1 - p2, accS[i][j][r] - p1 - p2);
+          } else {
+            acc = mk(accR[i][j][r], accI[i][j][r]);
+          }
+          cd v = alpha * acc;
           cd* p = C + gm * rsC + gn * csC;
           if (!BETA0) v += beta * (*p);
           *p = v;
         }
       }
+}
+
+// ORCWA_ZGEMM_3M=1 selects the 3-multiplication products (read once)
+static bool zgemm_gauss3() {
+  static const bool on = [] {
+    const char* e = getenv("ORCWA_ZGEMM_3M");
+    return e && e[0] == '1';
+  }();
+  return on;
 }
 
 void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
@@ -116,12 +178,22 @@ void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
                    int batch) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   dim3 grid((M + 63) / 64, (N + 63) / 64, batch);
-  if (beta.re == 0.0 && beta.im == 0.0)
-    hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA, rsA,
-                       csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
-  else
-    hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA,
-                       rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  const bool b0 = (beta.re == 0.0 && beta.im == 0.0);
+  if (zgemm_gauss3()) {
+    if (b0)
+      hipLaunchKernelGGL((k_zgemm<true, true>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
+                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+    else
+      hipLaunchKernelGGL((k_zgemm<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
+                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  } else {
+    if (b0)
+      hipLaunchKernelGGL((k_zgemm<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
+                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+    else
+      hipLaunchKernelGGL((k_zgemm<false, false>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
+                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -294,34 +366,52 @@ void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,
   }
 }
 
+// Two-level blocking: 32-row triangular blocks inside 128-row outer blocks;
+// the updates inside an outer block touch its 128 rows only, and the rows
+// outside get one rank-128 GEMM update per outer block, so the right-hand
+// side streams through HBM n/128 times per sweep instead of n/32.
+static constexpr int OB = 128;
+
 void zgetrs_batched(hipStream_t s, int n, int nrhs, const cd* const* LU, const int* ipiv,
                     cd* const* B, int batch) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
   const cd* const* Bc = (const cd* const*)B;
+  const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0);
   // all row swaps, in order
   hipLaunchKernelGGL(k_laswp, dim3((nrhs + 127) / 128, batch), dim3(128), 0, s, n, nrhs, 0, n, 0,
                      nrhs, B, ipiv);
   // forward substitution with unit L
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int nb = (n - k0 < NB) ? (n - k0) : NB;
-    hipLaunchKernelGGL(k_trsm_block<true>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, s, n, k0,
-                       nb, 0, nrhs, LU, B);
-    const int rest = n - k0 - nb;
-    if (rest > 0)
-      zgemm_batched(s, rest, nrhs, nb, mk(-1.0, 0.0), LU, (long long)(k0 + nb) + (long long)k0 * n,
-                    1, n, 0, Bc, (long long)k0, 1, n, 0, mk(1.0, 0.0), B, (long long)(k0 + nb), 1,
-                    n, batch);
+  for (int o0 = 0; o0 < n; o0 += OB) {
+    const int oe = std::min(n, o0 + OB);
+    for (int k0 = o0; k0 < oe; k0 += NB) {
+      const int nb = std::min(NB, oe - k0);
+      hipLaunchKernelGGL(k_trsm_block<true>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, s, n,
+                         k0, nb, 0, nrhs, LU, B);
+      const int rin = oe - k0 - nb;
+      if (rin > 0)
+        zgemm_batched(s, rin, nrhs, nb, mone, LU, (long long)(k0 + nb) + (long long)k0 * n, 1, n,
+                      0, Bc, (long long)k0, 1, n, 0, one, B, (long long)(k0 + nb), 1, n, batch);
+    }
+    if (oe < n)
+      zgemm_batched(s, n - oe, nrhs, oe - o0, mone, LU, (long long)oe + (long long)o0 * n, 1, n, 0,
+                    Bc, (long long)o0, 1, n, 0, one, B, (long long)oe, 1, n, batch);
   }
   // backward substitution with U
-  const int nblk = (n + NB - 1) / NB;
-  for (int bk = nblk - 1; bk >= 0; --bk) {
-    const int k0 = bk * NB;
-    const int nb = (n - k0 < NB) ? (n - k0) : NB;
-    hipLaunchKernelGGL(k_trsm_block<false>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, s, n,
-                       k0, nb, 0, nrhs, LU, B);
-    if (k0 > 0)
-      zgemm_batched(s, k0, nrhs, nb, mk(-1.0, 0.0), LU, (long long)k0 * n, 1, n, 0, Bc,
-                    (long long)k0, 1, n, 0, mk(1.0, 0.0), B, 0, 1, n, batch);
+  const int nob = (n + OB - 1) / OB;
+  for (int ob = nob - 1; ob >= 0; --ob) {
+    const int o0 = ob * OB, oe = std::min(n, o0 + OB);
+    const int nib = (oe - o0 + NB - 1) / NB;
+    for (int ib = nib - 1; ib >= 0; --ib) {
+      const int k0 = o0 + ib * NB, nb = std::min(NB, oe - k0);
+      hipLaunchKernelGGL(k_trsm_block<false>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, s, n,
+                         k0, nb, 0, nrhs, LU, B);
+      if (k0 > o0)
+        zgemm_batched(s, k0 - o0, nrhs, nb, mone, LU, (long long)o0 + (long long)k0 * n, 1, n, 0,
+                      Bc, (long long)k0, 1, n, 0, one, B, (long long)o0, 1, n, batch);
+    }
+    if (o0 > 0)
+      zgemm_batched(s, o0, nrhs, oe - o0, mone, LU, (long long)o0 * n, 1, n, 0, Bc, (long long)o0,
+                    1, n, 0, one, B, 0, 1, n, batch);
   }
 }
 

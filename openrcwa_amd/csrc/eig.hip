@@ -12,6 +12,7 @@
 //      smin = max(ulp |t_jj|, smlnum) pivot guard), W = Z X, unit 2-norm columns
 //   5. the reference's acceptance test ||A W - W diag(l)||max / ||A||max <= 1e-8
 // One workgroup owns one matrix; matrices live in HBM (column-major).
+#include <algorithm>
 #include <cfloat>
 
 #include "engine.h"
@@ -138,6 +139,216 @@ __global__ __launch_bounds__(256) void k_hess(int n, cd* const* Ap, cd* tau_all)
   if (tid == 0) {
     if (n >= 2) tau[n - 2] = mk(0.0, 0.0);
     tau[n - 1] = mk(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1b. blocked Hessenberg reduction (zgehrd / zlahr2 order) for large n.
+//     Per block of HB columns [i0, i0+nb): k_hess_panel reduces the panel
+//     columns one by one, keeping the rest of the matrix at its block-start
+//     value and representing the block's transformation Q_b = I - V T V^H by
+//     V (explicit, n x HB), T (HB x HB) and Y = A V T (rows i0+1..n-1); the
+//     trailing matrix is then updated with batched MFMA GEMMs:
+//        Y(0:i0+1) = A(0:i0+1, i0+1:n) V T
+//        A(0:i0+1, i0+1:n)   -= Y(0:i0+1) V^H             (right, top rows)
+//        A(i0+1:n, i0+nb:n)  -= Y(i0+1:n) V(i0+nb:n)^H    (right)
+//        A(i0+1:n, i0+nb:n)  -= V T^H (V^H A(i0+1:n, i0+nb:n))   (left)
+//     so each trailing element is read once per column (the matvec A v) and
+//     twice per block, instead of twice and written once per column.
+//     Workspace per matrix (X): V | Y | Ytmp (n x HB each) | T (HB x HB);
+//     (Z): W1 | W2 (HB x n each).
+// ---------------------------------------------------------------------------
+constexpr int HB = 32;
+
+__global__ __launch_bounds__(256) void k_hess_panel(int n, int i0, int nb, cd* const* Ap,
+                                                    cd* tau_all, cd* const* Xp) {
+  extern __shared__ cd sm[];
+  cd* sb = sm;          // current column, rows i0+1..n-1
+  cd* sv = sm + n;      // current reflector
+  __shared__ cd Ts[HB][HB + 1];
+  __shared__ cd s_w[HB], s_t[HB], s_row[HB];
+  __shared__ double sred[4];
+  __shared__ cd s_tau, s_scale;
+
+  const int mat = blockIdx.x;
+  cd* __restrict__ A = Ap[mat];
+  cd* __restrict__ V = Xp[mat];
+  cd* __restrict__ Y = V + (long long)n * HB;
+  cd* __restrict__ Tg = V + 3LL * n * HB;
+  cd* tau = tau_all + (long long)mat * n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = i0 + 1;
+
+  for (int j = 0; j < nb; ++j) {
+    const int c = i0 + j;
+    const long long cc = (long long)c * n;
+    for (int r = r0 + tid; r < n; r += 256) sb[r] = A[r + cc];
+    // the reflector below reads sb with another row-to-thread mapping
+    __syncthreads();
+    if (j > 0) {
+      // b -= Y(:, 0:j) conj(V(c, 0:j))
+      if (tid < j) s_row[tid] = conj(V[c + (long long)tid * n]);
+      __syncthreads();
+      for (int r = r0 + tid; r < n; r += 256) {
+        cd acc = mk(0.0, 0.0);
+        for (int p = 0; p < j; ++p) acc += Y[r + (long long)p * n] * s_row[p];
+        sb[r] -= acc;
+      }
+      __syncthreads();
+      // b <- (I - V T^H V^H) b
+      for (int p = wave; p < j; p += 4) {
+        const cd* vp = V + (long long)p * n;
+        double ar = 0.0, ai = 0.0;
+        for (int r = r0 + lane; r < n; r += 64) {
+          const cd z = cmul(vp[r], sb[r]);
+          ar += z.re;
+          ai += z.im;
+        }
+        ar = wave_sum(ar);
+        ai = wave_sum(ai);
+        if (lane == 0) s_w[p] = mk(ar, ai);
+      }
+      __syncthreads();
+      if (tid < j) {
+        cd acc = mk(0.0, 0.0);
+        for (int q = 0; q <= tid; ++q) acc += conj(Ts[q][tid]) * s_w[q];
+        s_t[tid] = acc;
+      }
+      __syncthreads();
+      for (int r = r0 + tid; r < n; r += 256) {
+        cd acc = mk(0.0, 0.0);
+        for (int p = 0; p < j; ++p) acc += V[r + (long long)p * n] * s_t[p];
+        sb[r] -= acc;
+      }
+      __syncthreads();
+    }
+    // reflector of b(c+1 .. n-1) (zlarfg: H^H x = beta e1)
+    double part = 0.0;
+    for (int r = c + 2 + tid; r < n; r += 256) part += norm2(sb[r]);
+    const double xn2 = block_sum(part, sred);
+    if (tid == 0) {
+      const cd alpha = sb[c + 1];
+      cd beta = alpha;
+      if (xn2 == 0.0 && alpha.im == 0.0) {
+        s_tau = mk(0.0, 0.0);
+        s_scale = mk(0.0, 0.0);
+      } else {
+        const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+        const double bt = (alpha.re >= 0.0) ? -nrm : nrm;
+        s_tau = mk((bt - alpha.re) / bt, -alpha.im / bt);
+        s_scale = cinv(alpha - mk(bt, 0.0));
+        beta = mk(bt, 0.0);
+      }
+      tau[c] = s_tau;
+      sb[c + 1] = beta;
+    }
+    __syncthreads();
+    const cd t = s_tau, sc = s_scale;
+    cd* Vj = V + (long long)j * n;
+    for (int r = tid; r < n; r += 256) {
+      cd v = mk(0.0, 0.0);
+      if (r == c + 1) v = mk(1.0, 0.0);
+      else if (r > c + 1) v = sb[r] * sc;
+      sv[r] = v;
+      Vj[r] = v;
+      // column c of the reduced matrix: updated entries above the
+      // subdiagonal, beta on it, the reflector below it
+      if (r >= r0) A[r + cc] = (r > c + 1) ? v : sb[r];
+    }
+    __syncthreads();
+    // t = V(:, 0:j)^H v
+    for (int p = wave; p < j; p += 4) {
+      const cd* vp = V + (long long)p * n;
+      double ar = 0.0, ai = 0.0;
+      for (int r = c + 1 + lane; r < n; r += 64) {
+        const cd z = cmul(vp[r], sv[r]);
+        ar += z.re;
+        ai += z.im;
+      }
+      ar = wave_sum(ar);
+      ai = wave_sum(ai);
+      if (lane == 0) s_t[p] = mk(ar, ai);
+    }
+    // y = A(r0:n, c+1:n) v(c+1:n): block-start columns (thread per row)
+    const int ra = r0 + tid, rb = r0 + tid + 256;
+    cd ya = mk(0.0, 0.0), yb = mk(0.0, 0.0);
+    if (ra < n) {
+      const bool hb = rb < n;
+#pragma unroll 4
+      for (int q = c + 1; q < n; ++q) {
+        const cd vq = sv[q];
+        const cd* col = A + (long long)q * n;
+        ya += col[ra] * vq;
+        if (hb) yb += col[rb] * vq;
+      }
+    }
+    __syncthreads();
+    // Y(:, j) = tau (y - Y(:, 0:j) t)
+    if (ra < n) {
+      cd acc = ya;
+      for (int p = 0; p < j; ++p) acc -= Y[ra + (long long)p * n] * s_t[p];
+      Y[ra + (long long)j * n] = t * acc;
+    }
+    if (rb < n) {
+      cd acc = yb;
+      for (int p = 0; p < j; ++p) acc -= Y[rb + (long long)p * n] * s_t[p];
+      Y[rb + (long long)j * n] = t * acc;
+    }
+    // T(0:j, j) = -tau T(0:j, 0:j) t,  T(j, j) = tau
+    if (tid < j) {
+      cd acc = mk(0.0, 0.0);
+      for (int q = tid; q < j; ++q) acc += Ts[tid][q] * s_t[q];
+      Ts[tid][j] = -(t * acc);
+    }
+    if (tid == j) Ts[j][j] = t;
+    __syncthreads();
+  }
+  for (int e = tid; e < HB * HB; e += 256) {
+    const int r = e % HB, q = e / HB;
+    Tg[e] = (r <= q && q < nb) ? Ts[r][q] : mk(0.0, 0.0);
+  }
+  // columns outside the panel of V stay zero for the GEMMs
+  for (long long e = (long long)nb * n + tid; e < (long long)HB * n; e += 256) V[e] = mk(0.0, 0.0);
+  if (tid == 0 && i0 + nb == n - 2) {
+    tau[n - 2] = mk(0.0, 0.0);
+    tau[n - 1] = mk(0.0, 0.0);
+  }
+}
+
+static void hess_blocked(hipStream_t s, int n, cd* const* A, cd* tau, cd* const* X,
+                         cd* const* Z, int batch) {
+  const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0), zero = mk(0.0, 0.0);
+  const cd* const* Ac = (const cd* const*)A;
+  const cd* const* Xc = (const cd* const*)X;
+  const cd* const* Zc = (const cd* const*)Z;
+  const long long oV = 0, oY = (long long)n * HB, oYt = 2LL * n * HB, oT = 3LL * n * HB;
+  const long long oW1 = 0, oW2 = (long long)HB * n;
+  for (int i0 = 0; i0 + 2 < n; i0 += HB) {
+    const int nb = std::min(HB, n - 2 - i0);
+    const int top = i0 + 1, m = n - i0 - 1, nt = n - i0 - nb;
+    hipLaunchKernelGGL(k_hess_panel, dim3(batch), dim3(256), 2 * n * sizeof(cd), s, n, i0, nb,
+                       A, tau, X);
+    // Ytmp(0:top) = A(0:top, i0+1:n) V(i0+1:n, :);  Y(0:top) = Ytmp T
+    zgemm_batched(s, top, nb, m, one, Ac, (long long)top * n, 1, n, 0, Xc, oV + top, 1, n, 0,
+                  zero, X, oYt, 1, n, batch);
+    zgemm_batched(s, top, nb, nb, one, Xc, oYt, 1, n, 0, Xc, oT, 1, HB, 0, zero, X, oY, 1, n,
+                  batch);
+    // A(0:top, i0+1:n) -= Y(0:top) V(i0+1:n)^H
+    zgemm_batched(s, top, m, nb, mone, Xc, oY, 1, n, 0, Xc, oV + top, n, 1, 1, one, A,
+                  (long long)top * n, 1, n, batch);
+    if (nt > 0) {
+      // A(top:n, i0+nb:n) -= Y(top:n) V(i0+nb:n)^H
+      zgemm_batched(s, m, nt, nb, mone, Xc, oY + top, 1, n, 0, Xc, oV + i0 + nb, n, 1, 1, one, A,
+                    (long long)top + (long long)(i0 + nb) * n, 1, n, batch);
+      // W1 = V(top:n)^H A(top:n, i0+nb:n);  W2 = T^H W1;  A -= V W2
+      zgemm_batched(s, nb, nt, m, one, Xc, oV + top, n, 1, 1, Ac,
+                    (long long)top + (long long)(i0 + nb) * n, 1, n, 0, zero, Z, oW1, 1, HB,
+                    batch);
+      zgemm_batched(s, nb, nt, nb, one, Xc, oT, HB, 1, 1, Zc, oW1, 1, HB, 0, zero, Z, oW2, 1, HB,
+                    batch);
+      zgemm_batched(s, m, nt, nb, mone, Xc, oV + top, 1, n, 0, Zc, oW2, 1, HB, 0, one, A,
+                    (long long)top + (long long)(i0 + nb) * n, 1, n, batch);
+    }
   }
 }
 
@@ -440,6 +651,7 @@ constexpr int QW = 64;
 constexpr int QLD = QW + 1;   // padded LDS leading dimension (bank spread)
 constexpr int QNS = 16;
 constexpr int QSCHED = QNS * QW;
+constexpr int AED_NW = 48;   // early-deflation window
 
 struct QrShared {
   cd Hw[QLD * QLD];     // window / small block (column-major, ld QLD)
@@ -453,6 +665,7 @@ struct QrShared {
   int bp[QNS];         // next pair of each bulge (-1: not introduced)
   int bdone[QNS];
   cd c, s, sh;
+  cd hv[QW];           // Householder vector (early deflation)
   int nsched, npass, l, ok, ired[4];
 };
 
@@ -709,6 +922,159 @@ __device__ __forceinline__ void ms_apply_window(const QrShared& S, cd* H, cd* Z,
   ms_apply_u_cols(S, Z, n, 0, n, a, wa);
 }
 
+// [c s; -conj(s) c] [f; g] = [r; 0] with c real (zlartg)
+__device__ inline void zlartg_dev(cd f, cd g, double& c, cd& sn) {
+  if (g.re == 0.0 && g.im == 0.0) { c = 1.0; sn = mk(0.0, 0.0); return; }
+  if (f.re == 0.0 && f.im == 0.0) { c = 0.0; sn = (1.0 / cabs(g)) * conj(g); return; }
+  const double af = cabs(f), ag = cabs(g), nrm = hypot(af, ag);
+  c = af / nrm;
+  sn = ((1.0 / (af * nrm)) * f) * conj(g);
+}
+
+// Householder reflector of x (zlarfg): H^H x = beta e1, H = I - tau v v^H, v0 = 1.
+// Wave 0, lanes own the entries x[i] = B[(r0 + i) + col * QLD], i < len; the
+// reflector is written to hv and the column to (beta, 0, ...).
+__device__ inline cd wave_house_col(cd* B, int r0, int col, int len, cd* hv) {
+  const int lane = threadIdx.x & 63;
+  const cd alpha = B[r0 + col * QLD];
+  double p = (lane >= 1 && lane < len) ? norm2(B[(r0 + lane) + col * QLD]) : 0.0;
+  const double xn2 = wave_sum(p);
+  cd tau = mk(0.0, 0.0), scale = mk(0.0, 0.0), beta = alpha;
+  if (!(xn2 == 0.0 && alpha.im == 0.0)) {
+    const double nrm = sqrt(norm2(alpha) + xn2);
+    const double bt = (alpha.re >= 0.0) ? -nrm : nrm;
+    tau = mk((bt - alpha.re) / bt, -alpha.im / bt);
+    scale = cinv(alpha - mk(bt, 0.0));
+    beta = mk(bt, 0.0);
+  }
+  if (lane < len) hv[lane] = (lane == 0) ? mk(1.0, 0.0) : B[(r0 + lane) + col * QLD] * scale;
+  wave_fence();
+  if (lane < len) B[(r0 + lane) + col * QLD] = (lane == 0) ? beta : mk(0.0, 0.0);
+  wave_fence();
+  return tau;
+}
+
+// T(r0:r0+len, c0:nw) <- (I - tau v v^H)^H T(...) and T(0:nw, r0:r0+len), V(...) <- (...)(I - tau v v^H)
+__device__ inline void wave_house_apply(cd* T, cd* V, int nw, int r0, int len, int c0, cd tau,
+                                        const cd* hv) {
+  const int lane = threadIdx.x & 63;
+  if (lane >= c0 && lane < nw) {
+    cd d = mk(0.0, 0.0);
+    for (int i = 0; i < len; ++i) d += cmul(hv[i], T[(r0 + i) + lane * QLD]);
+    const cd f = conj(tau) * d;
+    for (int i = 0; i < len; ++i) T[(r0 + i) + lane * QLD] -= hv[i] * f;
+  }
+  wave_fence();
+  if (lane < nw) {
+    cd d = mk(0.0, 0.0), e = mk(0.0, 0.0);
+    for (int j = 0; j < len; ++j) {
+      d += T[lane + (r0 + j) * QLD] * hv[j];
+      e += V[lane + (r0 + j) * QLD] * hv[j];
+    }
+    const cd fd = tau * d, fe = tau * e;
+    for (int j = 0; j < len; ++j) {
+      T[lane + (r0 + j) * QLD] -= fd * conj(hv[j]);
+      V[lane + (r0 + j) * QLD] -= fe * conj(hv[j]);
+    }
+  }
+  wave_fence();
+}
+
+// Aggressive early deflation (zlaqr3 mathematics) on the trailing nw x nw
+// window [kw, hi] of the active block [l, hi], kw > l: the window's Schur form
+// T = V^H H_w V in LDS; its spike s conj(V(0, :)) (s = H(kw, kw-1)) is tested
+// from the bottom, an undeflatable eigenvalue being moved to the top of the
+// window by swaps of adjacent diagonal entries; the undeflated part is then
+// returned to Hessenberg form and V is applied to the rest of H and to Z.
+// Returns the number of deflated eigenvalues, or -1 when the window's QR did
+// not converge (H untouched).  S.ok = 1 and S.shifts = the bottom QNS
+// undeflated eigenvalues when there are at least QNS of them.
+__device__ int aed_window(QrShared& S, cd* H, cd* Z, int n, int hi, int nw, double smlnum) {
+  const int kw = hi - nw + 1;
+  const int tid = threadIdx.x, lane = tid & 63;
+  ms_load_window(S, H, n, kw, nw);
+  QPROF_T0(tq0);
+  if (!lds_small_qr(S, S.Hw, nw, true, 30 * nw + 100)) return -1;
+  QPROF_ADD(15, tq0);
+  QPROF_T0(tq1);
+  const cd s = H[kw + (long long)(kw - 1) * n];
+  cd* T = S.Hw;
+  cd* V = S.U;
+  if (tid < 64) {
+    const double as = cabs1(s);
+    int ns = nw, ilst = 0;
+    for (int cnt = 0; cnt < nw; ++cnt) {
+      double foo = cabs1(T[(ns - 1) + (ns - 1) * QLD]);
+      if (foo == 0.0) foo = as;
+      if (as * cabs1(V[(ns - 1) * QLD]) <= fmax(smlnum, kUlp * foo)) {
+        --ns;
+      } else {
+        for (int k = ns - 2; k >= ilst; --k) {
+          const cd t11 = T[k + k * QLD], t22 = T[(k + 1) + (k + 1) * QLD];
+          double c;
+          cd sn;
+          zlartg_dev(T[k + (k + 1) * QLD], t22 - t11, c, sn);
+          if (lane >= k + 2 && lane < nw) {
+            const cd x = T[k + lane * QLD], y = T[(k + 1) + lane * QLD];
+            T[k + lane * QLD] = c * x + sn * y;
+            T[(k + 1) + lane * QLD] = c * y - conj(sn) * x;
+          }
+          if (lane < k) {
+            const cd x = T[lane + k * QLD], y = T[lane + (k + 1) * QLD];
+            T[lane + k * QLD] = c * x + conj(sn) * y;
+            T[lane + (k + 1) * QLD] = c * y - sn * x;
+          }
+          if (lane < nw) {
+            const cd x = V[lane + k * QLD], y = V[lane + (k + 1) * QLD];
+            V[lane + k * QLD] = c * x + conj(sn) * y;
+            V[lane + (k + 1) * QLD] = c * y - sn * x;
+          }
+          if (lane == 0) {
+            T[k + k * QLD] = t22;
+            T[(k + 1) + (k + 1) * QLD] = t11;
+          }
+          wave_fence();
+        }
+        ++ilst;
+      }
+    }
+    if (ns >= QNS) {
+      if (lane < QNS) S.shifts[lane] = T[(ns - QNS + lane) + (ns - QNS + lane) * QLD];
+    }
+    wave_fence();
+    const cd sp = (ns == 0) ? mk(0.0, 0.0) : s;
+    if (ns < nw || (sp.re == 0.0 && sp.im == 0.0)) {
+      if (ns > 1 && !(sp.re == 0.0 && sp.im == 0.0)) {
+        // reflect the spike s conj(V(0, 0:ns)) onto its first entry (the
+        // reflector is built in column QLD-1 of the LDS array, outside every
+        // window), then return T(0:ns, 0:ns) to Hessenberg form
+        if (lane < ns) T[lane + (QLD - 1) * QLD] = sp * conj(V[lane * QLD]);
+        wave_fence();
+        const cd tau = wave_house_col(T, 0, QLD - 1, ns, S.hv);
+        wave_house_apply(T, V, nw, 0, ns, 0, tau, S.hv);
+        for (int k = 0; k + 2 < ns; ++k) {
+          const cd t2 = wave_house_col(T, k + 1, k, ns - k - 1, S.hv);
+          wave_house_apply(T, V, nw, k + 1, ns - k - 1, k + 1, t2, S.hv);
+        }
+      }
+      if (lane == 0) S.c = sp * conj(V[0]);   // new spike H(kw, kw-1)
+      if (lane == 0) S.nsched = ns;
+    } else if (lane == 0) {
+      S.nsched = -nw;    // nothing deflated: H unchanged
+    }
+    if (lane == 0) S.ok = (ns >= QNS) ? 1 : 0;
+  }
+  __syncthreads();
+  QPROF_ADD(7, tq1);
+  const int code = S.nsched;
+  if (code < 0) return 0;
+  ms_store_window(S, H, n, kw, nw);
+  if (tid == 0) H[kw + (long long)(kw - 1) * n] = S.c;
+  ms_apply_window(S, H, Z, n, kw, nw);
+  __syncthreads();
+  return nw - code;
+}
+
 __global__ __launch_bounds__(256) void k_qr_ms(int n, cd* const* Hp, cd* const* Zp, int* info,
                                                int maxsweep) {
   __shared__ QrShared S;
@@ -753,21 +1119,42 @@ __global__ __launch_bounds__(256) void k_qr_ms(int n, cd* const* Hp, cd* const* 
       noprog = 0;
       continue;
     }
-    QPROF_T0(tp2);
-    // shifts: eigenvalues of the trailing QNS x QNS block (solved in S.U)
-    constexpr int ns = QNS;
-    for (int e = tid; e < QW * QW; e += 256) {
-      const int i = e % QW, j = e / QW;
-      cd v = mk(0.0, 0.0);
-      if (i < ns && j < ns && i <= j + 1)
-        v = H[(hi - ns + 1 + i) + (long long)(hi - ns + 1 + j) * n];
-      S.U[i + j * QLD] = v;
+    // aggressive early deflation of the trailing window; a sweep follows
+    // unless it deflated more than 14 % of the window
+    bool aed_shifts = false;
+    {
+      QPROF_T0(tpa);
+      const int nd = aed_window(S, H, Z, n, hi, AED_NW, smlnum);
+      QPROF_ADD(14, tpa);
+      QPROF_CNT(12);
+#ifdef ORCWA_QR_PROFILE
+      if (threadIdx.x == 0 && blockIdx.x == 0 && nd > 0) g_qr_prof[13] += nd;
+#endif
+      if (nd > 0) {
+        hi -= nd;
+        noprog = 0;
+        if (100 * nd > 14 * AED_NW || hi - l + 1 <= QW) continue;
+      }
+      aed_shifts = (nd >= 0) && (S.ok != 0);
     }
-    __syncthreads();
-    lds_small_qr(S, S.U, ns, false, 30 * ns + 100);
+    QPROF_T0(tp2);
+    // shifts: the bottom undeflated eigenvalues of the early-deflation
+    // window, else the eigenvalues of the trailing QNS x QNS block (in S.U)
+    constexpr int ns = QNS;
+    if (!aed_shifts) {
+      for (int e = tid; e < QW * QW; e += 256) {
+        const int i = e % QW, j = e / QW;
+        cd v = mk(0.0, 0.0);
+        if (i < ns && j < ns && i <= j + 1)
+          v = H[(hi - ns + 1 + i) + (long long)(hi - ns + 1 + j) * n];
+        S.U[i + j * QLD] = v;
+      }
+      __syncthreads();
+      lds_small_qr(S, S.U, ns, false, 30 * ns + 100);
+    }
     ++noprog;
     if (tid < ns) {
-      cd sh = S.U[tid + tid * QLD];
+      cd sh = aed_shifts ? S.shifts[tid] : S.U[tid + tid * QLD];
       if (noprog % 6 == 0) {
         const double pert = 0.75 * cabs1(H[hi + (long long)(hi - 1) * n]);
         double sn, cs;
@@ -956,6 +1343,68 @@ __global__ __launch_bounds__(256) void k_trevc(int n, const cd* const* Tp, cd* c
   }
 }
 
+// Blocked form of the same recurrence, by row blocks [i0, i1) of 32 from the
+// bottom: with X = I initially,
+//   X(i0:i1, i1:n) = -T(i0:i1, i1:n) X(i1:n, i1:n)          (one batched GEMM)
+// then, per column j >= i0, the rows k = min(j, i1)-1 .. i0 of the block:
+//   x(k) = (x(k) - sum_{m=k+1}^{i1-1} T(k, m) x(m)) / (T(k,k) - T(j,j))
+// (same smin guard), with the 32 x 32 diagonal block of T in LDS and the
+// block of column j in registers.
+constexpr int TB = 32;
+
+__global__ __launch_bounds__(256) void k_trevc_block(int n, int i0, int i1, const cd* const* Tp,
+                                                     cd* const* Xp) {
+  __shared__ cd Tb[TB][TB + 1];
+  const int b = blockIdx.y;
+  const cd* __restrict__ T = Tp[b];
+  cd* __restrict__ X = Xp[b];
+  const int nb = i1 - i0;
+  for (int e = threadIdx.x; e < TB * TB; e += 256) {
+    const int r = e % TB, c = e / TB;
+    Tb[r][c] = (r < nb && c < nb) ? T[(i0 + r) + (long long)(i0 + c) * n] : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  const int j = i0 + blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const double smlnum = kSafmin * ((double)n / kUlp);
+  const cd lam = T[j + (long long)j * n];
+  const double smin = fmax(kUlp * cabs1(lam), smlnum);
+  const int kend = min(j, i1) - i0;       // rows [0, kend) of the block are solved
+  cd* xc = X + (long long)j * n + i0;
+  cd x[TB];
+#pragma unroll
+  for (int r = 0; r < TB; ++r) x[r] = (r < nb) ? xc[r] : mk(0.0, 0.0);
+#pragma unroll
+  for (int kk = TB - 1; kk >= 0; --kk) {
+    if (kk < kend) {
+      cd acc = x[kk];
+#pragma unroll
+      for (int m = kk + 1; m < TB; ++m) acc -= Tb[kk][m] * x[m];
+      cd d = Tb[kk][kk] - lam;
+      if (cabs1(d) < smin) d = mk(smin, 0.0);
+      x[kk] = cdiv(acc, d);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < TB; ++r)
+    if (r < kend) xc[r] = x[r];
+}
+
+static void trevc_blocked(hipStream_t s, int n, const cd* const* T, cd* const* X, int batch) {
+  set_identity_batched(s, n, X, batch);
+  const int nblk = (n + TB - 1) / TB;
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int i0 = bk * TB, i1 = std::min(n, i0 + TB);
+    if (i1 < n)
+      zgemm_batched(s, i1 - i0, n - i1, n - i1, mk(-1.0, 0.0), T,
+                    (long long)i0 + (long long)i1 * n, 1, n, 0, (const cd* const*)X,
+                    (long long)i1 + (long long)i1 * n, 1, n, 0, mk(0.0, 0.0), X,
+                    (long long)i0 + (long long)i1 * n, 1, n, batch);
+    hipLaunchKernelGGL(k_trevc_block, dim3((n - i0 + 255) / 256, batch), dim3(256), 0, s, n, i0,
+                       i1, T, X);
+  }
+}
+
 // lam[j] = T[j,j]; unit 2-norm columns of W; relative residual.
 __global__ __launch_bounds__(256) void k_eig_finish(int n, const cd* const* Tp, cd* const* Wp,
                                                     cd* lam) {
@@ -1035,7 +1484,10 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
   if (batch <= 0 || n <= 0) return;
   const cd* const* Ac = (const cd* const*)A;
   if (n >= 3) {
-    hipLaunchKernelGGL(k_hess, dim3(batch), dim3(256), 3 * n * sizeof(cd), s, n, A, ws.tau);
+    if (n >= 128)
+      hess_blocked(s, n, A, ws.tau, ws.X, ws.Z, batch);
+    else
+      hipLaunchKernelGGL(k_hess, dim3(batch), dim3(256), 3 * n * sizeof(cd), s, n, A, ws.tau);
     // blocked explicit Q into Z; X holds the dense reflectors, W the T factors
     // and the two GEMM intermediates until the eigenvectors are formed (W must
     // hold nblk QB^2 + 2 QB n entries: n >= 104 or so, else the unblocked loop)
@@ -1074,7 +1526,10 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
   hipLaunchKernelGGL(k_zero_below_subdiag, dim3(64, batch), dim3(256), 0, s, n, A);
   hipLaunchKernelGGL(k_qr_ms, dim3(batch), dim3(256), 0, s, n, A, ws.Z, info, 40 + 3 * n);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(256), 0, s, n, Ac, ws.X);
+  if (n >= 64)
+    trevc_blocked(s, n, Ac, ws.X, batch);
+  else
+    hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(256), 0, s, n, Ac, ws.X);
   zgemm_batched(s, n, n, n, mk(1.0, 0.0), (const cd* const*)ws.Z, 0, 1, n, 0,
                 (const cd* const*)ws.X, 0, 1, n, 0, mk(0.0, 0.0), W, 0, 1, n, batch);
   hipLaunchKernelGGL(k_eig_finish, dim3(batch), dim3(256), 0, s, n, Ac, W, lam);

@@ -47,12 +47,14 @@ struct Blob {
 struct StepGroup {
   int count = 0;                 // instances in the group
   // table offsets into the blob
-  size_t Wa = 0, Va = 0, Wb = 0, Vb = 0, binv_a = 0, E = 0, H = 0, LU = 0, inst = 0;
+  size_t Wa = 0, Wb = 0, Vb = 0, binv_a = 0, E = 0, H = 0, LU = 0, inst = 0;
+  size_t pp = 0, M = 0;          // patterned a: its ProbParam and fe33^-1
 };
 
 struct StepTables {
   StepGroup ua, pa;              // a homogeneous / a patterned
   size_t keff_b = 0, k0d = 0;    // per instance of the chunk
+  size_t keff_a = 0;             // per instance: keff of a patterned upper layer, else null
 };
 
 struct ChunkTables {
@@ -623,12 +625,14 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
       std::vector<cd*> kb;
       std::vector<double> k0d;
       std::vector<cd*> uaWb, uaVb, uaBinv, uaE, uaH;
-      std::vector<cd*> paWa, paVa, paWb, paVb, paE, paH, paLU;
+      std::vector<cd*> paWa, paWb, paVb, paE, paH, paLU, paM, ka;
+      std::vector<ProbParam> paPP;
       std::vector<int> uaInst, paInst;
       for (int ii = 0; ii < ct.ninst; ++ii) {
         const int i = ct.i0 + ii;
         kb.push_back(Kof(ii, b));
         k0d.push_back(2.0 * kPi / p->lam[i] * p->thick[b]);
+        ka.push_back(pidx[(size_t)ii * L + a] >= 0 ? Kof(ii, a) : nullptr);
         if (pidx[(size_t)ii * L + a] < 0) {
           uaWb.push_back(Wof(ii, b));
           uaVb.push_back(Vof(ii, b));
@@ -638,17 +642,20 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
           uaInst.push_back(i);
         } else {
           paWa.push_back(Wof(ii, a));
-          paVa.push_back(Vof(ii, a));
           paWb.push_back(Wof(ii, b));
           paVb.push_back(Vof(ii, b));
           paE.push_back(islot(ii, 2));
           paH.push_back(islot(ii, 3));
           paLU.push_back(islot(ii, 4 + sidx));
           paInst.push_back(i);
+          const ProbParam& qa = pp[pidx[(size_t)ii * L + a]];
+          paPP.push_back(qa);
+          paM.push_back(p->gM[qa.grid]);
         }
       }
       stt.keff_b = B.put(kb);
       stt.k0d = B.put(k0d);
+      stt.keff_a = B.put(ka);
       stt.ua.count = (int)uaInst.size();
       stt.ua.Wb = B.put(uaWb);
       stt.ua.Vb = B.put(uaVb);
@@ -658,13 +665,14 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
       stt.ua.inst = B.put(uaInst);
       stt.pa.count = (int)paInst.size();
       stt.pa.Wa = B.put(paWa);
-      stt.pa.Va = B.put(paVa);
       stt.pa.Wb = B.put(paWb);
       stt.pa.Vb = B.put(paVb);
       stt.pa.E = B.put(paE);
       stt.pa.H = B.put(paH);
       stt.pa.LU = B.put(paLU);
       stt.pa.inst = B.put(paInst);
+      stt.pa.pp = B.put(paPP);
+      stt.pa.M = B.put(paM);
       ct.steps.push_back(stt);
     }
     std::vector<RtParam> rp(ct.ninst);
@@ -807,19 +815,23 @@ int orcwa_plan_execute(orcwa_plan* p) {
                            p->dev<const cd* const*>(q.Vb), p->dev<cd* const*>(q.H), q.count);
       }
       if (st.pa.count > 0) {
+        // one LU of W_a for both: [E | H'] = W_a^-1 [W_b | P_a V_b], H = K_a^-1 H'
+        // (E and H are adjacent slots, so they form one n x 2n right-hand side)
         const StepGroup& q = st.pa;
         cd* const* qLU = p->dev<cd* const*>(q.LU);
+        const ProbParam* qpp = p->dev<const ProbParam*>(q.pp);
+        const cd* const* qVb = p->dev<const cd* const*>(q.Vb);
+        launch_pv_in(s, g, qpp, qVb, qLU, q.count);
+        zgemm_batched(s, h, n, h, one, p->dev<const cd* const*>(q.M), 0, 1, h, 0, C(qLU), 0, 1, h,
+                      0, zero, qLU, (long long)h * n, 1, h, q.count);
+        launch_pv_out(s, g, qpp, C(qLU), (long long)h * n, qVb, p->dev<cd* const*>(q.H), q.count);
         copy_batched(s, nn, p->dev<const cd* const*>(q.Wa), qLU, q.count);
         copy_batched(s, nn, p->dev<const cd* const*>(q.Wb), p->dev<cd* const*>(q.E), q.count);
         zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
-        zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.E), q.count);
-        copy_batched(s, nn, p->dev<const cd* const*>(q.Va), qLU, q.count);
-        copy_batched(s, nn, p->dev<const cd* const*>(q.Vb), p->dev<cd* const*>(q.H), q.count);
-        zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
-        zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.H), q.count);
+        zgetrs_batched(s, n, 2 * n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.E), q.count);
       }
       // A = E + H (in E), B = E - H (in H)
-      launch_sum_diff(s, n, E, H, ni);
+      launch_sum_diff(s, n, E, H, p->dev<const cd* const*>(st.keff_a), ni);
       // R~ = phi R phi, kept transposed (the scaling is symmetric in i, j)
       cd* const* Rcur = p->dev<cd* const*>(ct.Rt[si & 1]);
       cd* const* Rnext = p->dev<cd* const*>(ct.Rt[(si + 1) & 1]);

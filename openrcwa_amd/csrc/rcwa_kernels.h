@@ -82,7 +82,12 @@ void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const doubl
 void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* rt,
                double* ref, double* trn, cd* amp, int* info, int max_pol, int batch);
 void launch_transpose(hipStream_t s, int n, const cd* const* src, cd* const* dst, int batch);
-void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, int batch);
+void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, const cd* const* keff_a,
+                     int batch);
+void launch_pv_in(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* Vb,
+                  cd* const* out, int batch);
+void launch_pv_out(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* U,
+                   long long offU, const cd* const* Vb, cd* const* H, int batch);
 void launch_cascade_tail(hipStream_t s, int n, int i0, const cd* const* LUt, const int* piv,
                          const cd* const* A, const cd* const* B, const cd* const* Rt, cd* rt,
                          int batch);

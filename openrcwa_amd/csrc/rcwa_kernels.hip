@@ -404,14 +404,65 @@ __global__ void k_transpose(int n, const cd* const* src, cd* const* dst) {
   }
 }
 
-__global__ void k_sum_diff(long long count, cd* const* Ep, cd* const* Hp) {
+// A = E + D H, B = E - D H with D = diag(1/keff_a) where the instance's
+// upper layer is patterned (H holds W_a^-1 P_a V_b there) and D = I otherwise.
+__global__ void k_sum_diff(int n, cd* const* Ep, cd* const* Hp, const cd* const* keff_a) {
   cd* E = Ep[blockIdx.y];
   cd* H = Hp[blockIdx.y];
+  const cd* kf = keff_a[blockIdx.y];
+  const long long count = (long long)n * n;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
        e += (long long)gridDim.x * blockDim.x) {
-    const cd a = E[e], c = H[e];
+    const cd a = E[e];
+    cd c = H[e];
+    if (kf) c = cdiv(c, kf[e % n]);
     E[e] = a + c;
     H[e] = a - c;
+  }
+}
+
+// Interface H = V_a^-1 V_b for a patterned upper layer a without inverting V_a:
+// P_a Q_a W_a = W_a K_a^2 and V_a = Q_a W_a K_a^-1 give V_a^-1 = K_a^-1 W_a^-1 P_a,
+// and with P = [[Kx M Ky, I - Kx M Kx], [Ky M Ky - I, -Ky M Kx]] (M = fe33^-1,
+// k0 units, rcwa/Layer.cpp:150-169):
+//   P V = [Kx U + V2; Ky U - V1],  U = M (Ky V1 - Kx V2),  V = [V1; V2].
+// k_pv_in writes Ky V1 - Kx V2 (h x n, ld h); k_pv_out forms P V from U.
+__global__ void k_pv_in(const ConvGeom g, const ProbParam* params, const cd* const* Vb,
+                        cd* const* out) {
+  const int b = blockIdx.y;
+  const ProbParam p = params[b];
+  const int h = g.nx * g.ny, n = 2 * h;
+  const cd* V = Vb[b];
+  cd* O = out[b];
+  const long long cnt = (long long)h * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < cnt;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % h);
+    const long long col = e / h;
+    double kx, ky;
+    k_vectors(g, p, r, kx, ky);
+    O[e] = ky * V[r + col * n] - kx * V[h + r + col * n];
+  }
+}
+
+__global__ void k_pv_out(const ConvGeom g, const ProbParam* params, const cd* const* Up,
+                         long long offU, const cd* const* Vb, cd* const* Hp) {
+  const int b = blockIdx.y;
+  const ProbParam p = params[b];
+  const int h = g.nx * g.ny, n = 2 * h;
+  const cd* U = Up[b] + offU;
+  const cd* V = Vb[b];
+  cd* H = Hp[b];
+  const long long cnt = (long long)n * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < cnt;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n);
+    const long long col = e / n;
+    const int rr = r < h ? r : r - h;
+    double kx, ky;
+    k_vectors(g, p, rr, kx, ky);
+    const cd u = U[rr + col * h];
+    H[e] = (r < h) ? kx * u + V[h + rr + col * n] : ky * u - V[rr + col * n];
   }
 }
 
@@ -738,11 +789,30 @@ void launch_transpose(hipStream_t s, int n, const cd* const* src, cd* const* dst
   hipLaunchKernelGGL(k_transpose, dim3(t, t, batch), dim3(256), 0, s, n, src, dst);
 }
 
-void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, int batch) {
+void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, const cd* const* keff_a,
+                     int batch) {
   if (batch <= 0) return;
   long long b = blocks_for((long long)n * n);
-  hipLaunchKernelGGL(k_sum_diff, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s,
-                     (long long)n * n, E, H);
+  hipLaunchKernelGGL(k_sum_diff, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s, n,
+                     E, H, keff_a);
+}
+
+void launch_pv_in(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* Vb,
+                  cd* const* out, int batch) {
+  if (batch <= 0) return;
+  const int h = g.nx * g.ny;
+  long long b = blocks_for(2LL * h * h);
+  hipLaunchKernelGGL(k_pv_in, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s, g,
+                     params, Vb, out);
+}
+
+void launch_pv_out(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* U,
+                   long long offU, const cd* const* Vb, cd* const* H, int batch) {
+  if (batch <= 0) return;
+  const int h = g.nx * g.ny;
+  long long b = blocks_for(4LL * h * h);
+  hipLaunchKernelGGL(k_pv_out, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s, g,
+                     params, U, offU, Vb, H);
 }
 
 void launch_cascade_tail(hipStream_t s, int n, int i0, const cd* const* LUt, const int* piv,

@@ -651,7 +651,7 @@ constexpr int QW = 64;
 constexpr int QLD = QW + 1;   // padded LDS leading dimension (bank spread)
 constexpr int QNS = 16;
 constexpr int QSCHED = QNS * QW;
-constexpr int AED_NW = 48;   // early-deflation window
+constexpr int AED_NW = 24;   // early-deflation window
 
 struct QrShared {
   cd Hw[QLD * QLD];     // window / small block (column-major, ld QLD)
@@ -922,15 +922,6 @@ __device__ __forceinline__ void ms_apply_window(const QrShared& S, cd* H, cd* Z,
   ms_apply_u_cols(S, Z, n, 0, n, a, wa);
 }
 
-// [c s; -conj(s) c] [f; g] = [r; 0] with c real (zlartg)
-__device__ inline void zlartg_dev(cd f, cd g, double& c, cd& sn) {
-  if (g.re == 0.0 && g.im == 0.0) { c = 1.0; sn = mk(0.0, 0.0); return; }
-  if (f.re == 0.0 && f.im == 0.0) { c = 0.0; sn = (1.0 / cabs(g)) * conj(g); return; }
-  const double af = cabs(f), ag = cabs(g), nrm = hypot(af, ag);
-  c = af / nrm;
-  sn = ((1.0 / (af * nrm)) * f) * conj(g);
-}
-
 // Householder reflector of x (zlarfg): H^H x = beta e1, H = I - tau v v^H, v0 = 1.
 // Wave 0, lanes own the entries x[i] = B[(r0 + i) + col * QLD], i < len; the
 // reflector is written to hv and the column to (beta, 0, ...).
@@ -980,12 +971,12 @@ __device__ inline void wave_house_apply(cd* T, cd* V, int nw, int r0, int len, i
   wave_fence();
 }
 
-// Aggressive early deflation (zlaqr3 mathematics) on the trailing nw x nw
-// window [kw, hi] of the active block [l, hi], kw > l: the window's Schur form
-// T = V^H H_w V in LDS; its spike s conj(V(0, :)) (s = H(kw, kw-1)) is tested
-// from the bottom, an undeflatable eigenvalue being moved to the top of the
-// window by swaps of adjacent diagonal entries; the undeflated part is then
-// returned to Hessenberg form and V is applied to the rest of H and to Z.
+// Aggressive early deflation (zlaqr3 mathematics, without its reordering) on
+// the trailing nw x nw window [kw, hi] of the active block [l, hi], kw > l:
+// the window's Schur form T = V^H H_w V in LDS; its spike s conj(V(0, :))
+// (s = H(kw, kw-1)) is tested from the bottom up to the first undeflatable
+// eigenvalue; the undeflated part is returned to Hessenberg form and V is
+// applied to the rest of H and to Z.
 // Returns the number of deflated eigenvalues, or -1 when the window's QR did
 // not converge (H untouched).  S.ok = 1 and S.shifts = the bottom QNS
 // undeflated eigenvalues when there are at least QNS of them.
@@ -1001,42 +992,15 @@ __device__ int aed_window(QrShared& S, cd* H, cd* Z, int n, int hi, int nw, doub
   cd* T = S.Hw;
   cd* V = S.U;
   if (tid < 64) {
+    // trailing eigenvalues whose spike entry is negligible deflate; the first
+    // undeflatable one from the bottom ends the test (no reordering)
     const double as = cabs1(s);
-    int ns = nw, ilst = 0;
-    for (int cnt = 0; cnt < nw; ++cnt) {
+    int ns = nw;
+    while (ns > 0) {
       double foo = cabs1(T[(ns - 1) + (ns - 1) * QLD]);
       if (foo == 0.0) foo = as;
-      if (as * cabs1(V[(ns - 1) * QLD]) <= fmax(smlnum, kUlp * foo)) {
-        --ns;
-      } else {
-        for (int k = ns - 2; k >= ilst; --k) {
-          const cd t11 = T[k + k * QLD], t22 = T[(k + 1) + (k + 1) * QLD];
-          double c;
-          cd sn;
-          zlartg_dev(T[k + (k + 1) * QLD], t22 - t11, c, sn);
-          if (lane >= k + 2 && lane < nw) {
-            const cd x = T[k + lane * QLD], y = T[(k + 1) + lane * QLD];
-            T[k + lane * QLD] = c * x + sn * y;
-            T[(k + 1) + lane * QLD] = c * y - conj(sn) * x;
-          }
-          if (lane < k) {
-            const cd x = T[lane + k * QLD], y = T[lane + (k + 1) * QLD];
-            T[lane + k * QLD] = c * x + conj(sn) * y;
-            T[lane + (k + 1) * QLD] = c * y - sn * x;
-          }
-          if (lane < nw) {
-            const cd x = V[lane + k * QLD], y = V[lane + (k + 1) * QLD];
-            V[lane + k * QLD] = c * x + conj(sn) * y;
-            V[lane + (k + 1) * QLD] = c * y - sn * x;
-          }
-          if (lane == 0) {
-            T[k + k * QLD] = t22;
-            T[(k + 1) + (k + 1) * QLD] = t11;
-          }
-          wave_fence();
-        }
-        ++ilst;
-      }
+      if (as * cabs1(V[(ns - 1) * QLD]) <= fmax(smlnum, kUlp * foo)) --ns;
+      else break;
     }
     if (ns >= QNS) {
       if (lane < QNS) S.shifts[lane] = T[(ns - QNS + lane) + (ns - QNS + lane) * QLD];

@@ -294,6 +294,96 @@ __global__ void k_laswp(int n, int ncols_total, int k0, int nb, int c0, int c1, 
   }
 }
 
+// The interchanges of one panel (rows k0..k0+nb-1 with their pivot rows,
+// applied in order) as a single gather: thread 0 composes them on the <= 2 nb
+// rows they touch (slot s holds row R[s]; after the swaps it holds the
+// original row R[src[s]]); the workgroup then stages those rows of its 16
+// columns in LDS and writes back the moved ones.  Columns [c0, c1).
+constexpr int LASWP_MAXS = 64, LASWP_NC = 16;
+__global__ __launch_bounds__(256) void k_laswp_panel(int n, int k0, int nb, int c0, int c1,
+                                                     cd* const* Ap, const int* ipiv) {
+  __shared__ int R[LASWP_MAXS], src[LASWP_MAXS];
+  __shared__ int s_ns;
+  __shared__ cd tile[LASWP_MAXS][LASWP_NC];
+  const int b = blockIdx.y;
+  const int* pv = ipiv + (long long)b * n;
+  if (threadIdx.x == 0) {
+    int ns = nb;
+    for (int j = 0; j < nb; ++j) { R[j] = k0 + j; src[j] = j; }
+    for (int j = 0; j < nb; ++j) {
+      const int p = pv[k0 + j];
+      int sp = -1;
+      if (p < k0 + nb) {
+        sp = p - k0;
+      } else {
+        for (int t = nb; t < ns; ++t)
+          if (R[t] == p) { sp = t; break; }
+        if (sp < 0) { sp = ns; R[ns] = p; src[ns] = ns; ++ns; }
+      }
+      const int t = src[j];
+      src[j] = src[sp];
+      src[sp] = t;
+    }
+    s_ns = ns;
+  }
+  __syncthreads();
+  const int ns = s_ns;
+  const int c = threadIdx.x % LASWP_NC, sg = threadIdx.x / LASWP_NC;
+  const int col = c0 + blockIdx.x * LASWP_NC + c;
+  cd* __restrict__ A = Ap[b] + (long long)col * n;
+  if (col < c1)
+    for (int t = sg; t < ns; t += 256 / LASWP_NC) tile[t][c] = A[R[t]];
+  __syncthreads();
+  if (col < c1)
+    for (int t = sg; t < ns; t += 256 / LASWP_NC)
+      if (src[t] != t) A[R[t]] = tile[src[t]][c];
+}
+
+// perm[r] = the original row that ends in row r after all n interchanges
+// (one thread per matrix, PERM_MB matrices per workgroup, in LDS)
+constexpr int PERM_MB = 16;
+__global__ __launch_bounds__(PERM_MB) void k_ipiv_to_perm(int n, const int* ipiv, int* perm,
+                                                          int batch) {
+  extern __shared__ int sp[];
+  const int b = blockIdx.x * PERM_MB + threadIdx.x;
+  int* q = sp + (long long)threadIdx.x * n;
+  if (b < batch) {
+    const int* pv = ipiv + (long long)b * n;
+    for (int r = 0; r < n; ++r) q[r] = r;
+    for (int k = 0; k < n; ++k) {
+      const int p = pv[k];
+      if (p != k) {
+        const int t = q[k];
+        q[k] = q[p];
+        q[p] = t;
+      }
+    }
+    int* out = perm + (long long)b * n;
+    for (int r = 0; r < n; ++r) out[r] = q[r];
+  }
+}
+
+// B <- P^T B as a gather through LDS: 8 columns per workgroup
+constexpr int PERM_NC = 8;
+__global__ __launch_bounds__(256) void k_apply_perm(int n, int nrhs, const int* perm,
+                                                    cd* const* Bp) {
+  extern __shared__ cd X[];   // [PERM_NC][n]
+  const int b = blockIdx.y;
+  const int c0 = blockIdx.x * PERM_NC;
+  cd* __restrict__ B = Bp[b];
+  const int* pm = perm + (long long)b * n;
+  const int nc = min(PERM_NC, nrhs - c0);
+  for (int e = threadIdx.x; e < nc * n; e += 256) {
+    const int c = e / n, r = e % n;
+    X[c * n + r] = B[r + (long long)(c0 + c) * n];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nc * n; e += 256) {
+    const int c = e / n, r = e % n;
+    B[r + (long long)(c0 + c) * n] = X[c * n + pm[r]];
+  }
+}
+
 // X <- L11^{-1} X for the nb x ncols block of rows [k0, k0+nb), columns
 // [c0, c0+ncols) of X (ld = n), L11 unit lower at (k0,k0) of L (ld = n).
 template <bool UNIT_LOWER>
@@ -350,12 +440,12 @@ void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,
                        info_map);
     // swaps on the columns left and right of the panel
     if (k0 > 0)
-      hipLaunchKernelGGL(k_laswp, dim3((k0 + 127) / 128, batch), dim3(128), 0, s, n, n, k0, nb, 0,
-                         k0, A, (const int*)ipiv);
+      hipLaunchKernelGGL(k_laswp_panel, dim3((k0 + LASWP_NC - 1) / LASWP_NC, batch), dim3(256), 0,
+                         s, n, k0, nb, 0, k0, A, (const int*)ipiv);
     const int rest = n - k0 - nb;
     if (rest > 0) {
-      hipLaunchKernelGGL(k_laswp, dim3((rest + 127) / 128, batch), dim3(128), 0, s, n, n, k0, nb,
-                         k0 + nb, n, A, (const int*)ipiv);
+      hipLaunchKernelGGL(k_laswp_panel, dim3((rest + LASWP_NC - 1) / LASWP_NC, batch), dim3(256),
+                         0, s, n, k0, nb, k0 + nb, n, A, (const int*)ipiv);
       hipLaunchKernelGGL(k_trsm_block<true>, dim3((rest + 255) / 256, batch), dim3(256), 0, s, n,
                          k0, nb, k0 + nb, rest, Ac, A);
       // A22 -= L21 * U12
@@ -373,13 +463,23 @@ void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,
 static constexpr int OB = 128;
 
 void zgetrs_batched(hipStream_t s, int n, int nrhs, const cd* const* LU, const int* ipiv,
-                    cd* const* B, int batch) {
+                    cd* const* B, int batch, int* perm_ws) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return;
   const cd* const* Bc = (const cd* const*)B;
   const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0);
-  // all row swaps, in order
-  hipLaunchKernelGGL(k_laswp, dim3((nrhs + 127) / 128, batch), dim3(128), 0, s, n, nrhs, 0, n, 0,
-                     nrhs, B, ipiv);
+  // all row interchanges: composed once per matrix and applied as a gather
+  // (when a workspace of batch x n ints is given), else swap by swap
+  const size_t perm_lds = (size_t)PERM_MB * n * sizeof(int);
+  const size_t gather_lds = (size_t)PERM_NC * n * sizeof(cd);
+  if (perm_ws && perm_lds <= 64 * 1024 && gather_lds <= 64 * 1024) {
+    hipLaunchKernelGGL(k_ipiv_to_perm, dim3((batch + PERM_MB - 1) / PERM_MB), dim3(PERM_MB),
+                       perm_lds, s, n, ipiv, perm_ws, batch);
+    hipLaunchKernelGGL(k_apply_perm, dim3((nrhs + PERM_NC - 1) / PERM_NC, batch), dim3(256),
+                       gather_lds, s, n, nrhs, (const int*)perm_ws, B);
+  } else {
+    hipLaunchKernelGGL(k_laswp, dim3((nrhs + 127) / 128, batch), dim3(128), 0, s, n, nrhs, 0, n, 0,
+                       nrhs, B, ipiv);
+  }
   // forward substitution with unit L
   for (int o0 = 0; o0 < n; o0 += OB) {
     const int oe = std::min(n, o0 + OB);

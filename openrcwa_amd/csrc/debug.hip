@@ -81,18 +81,22 @@ int orcwa_dbg_zgesv(orcwa_gpu_context* ctx, int n, int nrhs, const double* A, co
   int rc;
   if ((rc = a.init(s, (long long)n * n, batch, A)) || (rc = b.init(s, (long long)n * nrhs, batch, B)))
     return rc;
-  int *ipiv = nullptr, *dinfo = nullptr;
+  int *ipiv = nullptr, *dinfo = nullptr, *perm = nullptr;
   DCHK(hipMalloc(&ipiv, sizeof(int) * n * batch));
+  DCHK(hipMalloc(&perm, sizeof(int) * n * batch));
   DCHK(hipMalloc(&dinfo, sizeof(int) * batch));
   DCHK(hipMemsetAsync(dinfo, 0, sizeof(int) * batch, s));
   zgetrf_batched(s, n, a.table, ipiv, dinfo, nullptr, batch);
-  zgetrs_batched(s, n, nrhs, (const cd* const*)a.table, ipiv, b.table, batch);
+  // odd batches exercise the swap-by-swap path, even ones the gather
+  zgetrs_batched(s, n, nrhs, (const cd* const*)a.table, ipiv, b.table, batch,
+                 (batch % 2 == 0) ? perm : nullptr);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
     e = hipMemcpyAsync(X, b.data, sizeof(cd) * n * nrhs * batch, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(info, dinfo, sizeof(int) * batch, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(ipiv);
+  (void)hipFree(perm);
   (void)hipFree(dinfo);
   DCHK(e);
   return ORCWA_OK;

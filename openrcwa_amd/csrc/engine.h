@@ -28,9 +28,10 @@ void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,
                     const int* info_map, int batch);
 
 // Solve A X = B in place of B (n x nrhs, column-major, ldb = n) with the
-// factors of zgetrf_batched.
+// factors of zgetrf_batched.  perm_ws: optional batch x n ints of workspace
+// (the row interchanges are then applied as one gather).
 void zgetrs_batched(hipStream_t s, int n, int nrhs, const cd* const* LU, const int* ipiv,
-                    cd* const* B, int batch);
+                    cd* const* B, int batch, int* perm_ws = nullptr);
 
 // Elementwise helpers over batches of n x n column-major matrices.
 void set_identity_batched(hipStream_t s, int n, cd* const* A, int batch);

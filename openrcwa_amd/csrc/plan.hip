@@ -121,6 +121,7 @@ struct orcwa_plan {
   cd* d_tau = nullptr;             // [pmax*n]
   int* d_ipiv = nullptr;           // [max(chunk, pmax)*n] scratch pivots
   int* d_spiv = nullptr;           // [L-1][chunk*n] pivots of each cascade step
+  int* d_perm = nullptr;           // [max(chunk, pmax, G)*n] row permutations of a solve
   cd* d_rtv = nullptr;             // [chunk][4][n] Rud / Tdd columns i0, h+i0
   int* d_pinfo = nullptr;          // [nprob_total]
   double* d_resid = nullptr;       // [nprob_total]
@@ -409,6 +410,7 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   const size_t o_islots = take((size_t)chunk * islots * slot);
   const size_t o_spiv = take((size_t)(L - 1) * chunk * n * sizeof(int));
   const size_t o_rtv = take((size_t)chunk * 4 * n * sizeof(cd));
+  const size_t o_perm = take((size_t)std::max(std::max(chunk, PM), std::max(1, G)) * n * sizeof(int));
   const size_t o_lam = take((size_t)PM * n * sizeof(cd));
   const size_t o_keffp = take((size_t)PM * n * sizeof(cd));
   const size_t o_keffu = take((size_t)UM * n * sizeof(cd));
@@ -453,6 +455,7 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   p->d_ipiv = (int*)(W + o_ipiv);
   p->d_spiv = (int*)(W + o_spiv);
   p->d_rtv = (cd*)(W + o_rtv);
+  p->d_perm = (int*)(W + o_perm);
   p->d_pinfo = (int*)(W + o_pinfo);
   p->d_resid = (double*)(W + o_resid);
   p->d_ref = (double*)(W + o_ref);
@@ -741,7 +744,7 @@ int orcwa_plan_execute(orcwa_plan* p) {
     zgetrf_batched(s, h, p->dev<cd* const*>(p->t_gLU), p->d_gipiv, p->d_ginfo, nullptr, G);
     set_identity_batched(s, h, p->dev<cd* const*>(p->t_gM), G);
     zgetrs_batched(s, h, h, p->dev<const cd* const*>(p->t_gLU), p->d_gipiv,
-                   p->dev<cd* const*>(p->t_gM), G);
+                   p->dev<cd* const*>(p->t_gM), G, p->d_perm);
     const cd* const* Mt = p->dev<const cd* const*>(p->t_gM);
     zgemm_batched(s, h, h, h, one, Mt, 0, 1, h, 0, p->dev<const cd* const*>(p->t_gE11), 0, 1, h,
                   0, zero, p->dev<cd* const*>(p->t_gS1), 0, 1, h, G);
@@ -828,7 +831,8 @@ int orcwa_plan_execute(orcwa_plan* p) {
         copy_batched(s, nn, p->dev<const cd* const*>(q.Wa), qLU, q.count);
         copy_batched(s, nn, p->dev<const cd* const*>(q.Wb), p->dev<cd* const*>(q.E), q.count);
         zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
-        zgetrs_batched(s, n, 2 * n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.E), q.count);
+        zgetrs_batched(s, n, 2 * n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.E), q.count,
+                       p->d_perm);
       }
       // A = E + H (in E), B = E - H (in H)
       launch_sum_diff(s, n, E, H, p->dev<const cd* const*>(st.keff_a), ni);
@@ -853,7 +857,7 @@ int orcwa_plan_execute(orcwa_plan* p) {
       }
       zgetrf_batched(s, n, M, piv, p->d_info + ct.i0, nullptr, ni);
       // R_new^T = (F + G)^-T (F - G)^T
-      if (!last) zgetrs_batched(s, n, n, C(M), piv, Rnext, ni);
+      if (!last) zgetrs_batched(s, n, n, C(M), piv, Rnext, ni, p->d_perm);
     }
     {
       // columns i0, h+i0 of Rud and Tdd

@@ -41,10 +41,13 @@ def test_zgemm_ops(gpu_ctx, tA, cA, tB, cB):
     assert np.abs(out - opA @ opB).max() < 1e-12
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (5, 3), (31, 31), (32, 32), (33, 7), (100, 100),
-                                    (450, 450)])
-def test_zgesv(gpu_ctx, n, nrhs):
-    A, B = crand(2, n, n), crand(2, n, nrhs)
+@pytest.mark.parametrize("n,nrhs,batch", [(1, 1, 2), (5, 3, 2), (31, 31, 2), (32, 32, 2),
+                                          (33, 7, 2), (100, 100, 2), (450, 450, 2),
+                                          (97, 40, 3), (450, 900, 3)])
+def test_zgesv(gpu_ctx, n, nrhs, batch):
+    # even batches apply the row interchanges of the solve as one gather,
+    # odd ones swap by swap (both paths of zgetrs_batched)
+    A, B = crand(batch, n, n), crand(batch, n, nrhs)
     X, info = g.zgesv(gpu_ctx, A, B)
     assert not info.any()
     ref = np.linalg.solve(A, B)

@@ -18,10 +18,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // MFMA f64 16x16x4 lane layout (measured on gfx950): A[i][k] in lane i+16k,
 // B[k][j] in lane j+16k, D[i][j] in lane j+16(i%4), element i/4.
 // ---------------------------------------------------------------------------
-// GAUSS3: the 3-multiplication complex product (P1 = Ar Br, P2 = Ai Bi,
-// P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2): 3 MFMAs per
-// complex update instead of 4, with a normwise error of the same order.
-template <bool BETA0, bool GAUSS3>
+template <bool BETA0>
 __global__ __launch_bounds__(256) void k_zgemm(
     int M, int N, int K, cd alpha, cd beta,
     const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
@@ -68,14 +65,13 @@ __global__ __launch_bounds__(256) void k_zgemm(
     rb[j] = (bok[j] && bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
   }
 
-  v4d accR[2][2], accI[2][2], accS[2][2];
+  v4d accR[2][2], accI[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
       accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-      accS[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
 
   for (int k0 = 0; k0 < K; k0 += 16) {
@@ -108,32 +104,15 @@ __global__ __launch_bounds__(256) void k_zgemm(
         br[i] = sBr[ka][wn + 16 * i + (lane & 15)];
         bi[i] = sBi[ka][wn + 16 * i + (lane & 15)];
       }
-      if (GAUSS3) {
-        double as[2], bs[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          as[i] = ar[i] + ai[i];
-          bs[i] = br[i] + bi[i];
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], accI[i][j], 0, 0, 0);
-            accS[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], accS[i][j], 0, 0, 0);
-          }
-      } else {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
-            accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
-          }
-      }
     }
     __syncthreads();
   }
@@ -147,28 +126,13 @@ __global__ __launch_bounds__(256) void k_zgemm(
         const int gm = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
         const int gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N) {
-          cd acc;
-          if (GAUSS3) {
-            const double p1 = accR[i][j][r], p2 = accI[i][j][r];
-            acc = mk(p1 - p2, accS[i][j][r] - p1 - p2);
-          } else {
-            acc = mk(accR[i][j][r], accI[i][j][r]);
-          }
+          const cd acc = mk(accR[i][j][r], accI[i][j][r]);
           cd v = alpha * acc;
           cd* p = C + gm * rsC + gn * csC;
           if (!BETA0) v += beta * (*p);
           *p = v;
         }
       }
-}
-
-// ORCWA_ZGEMM_3M=1 selects the 3-multiplication products (read once)
-static bool zgemm_gauss3() {
-  static const bool on = [] {
-    const char* e = getenv("ORCWA_ZGEMM_3M");
-    return e && e[0] == '1';
-  }();
-  return on;
 }
 
 void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
@@ -178,22 +142,12 @@ void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
                    int batch) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   dim3 grid((M + 63) / 64, (N + 63) / 64, batch);
-  const bool b0 = (beta.re == 0.0 && beta.im == 0.0);
-  if (zgemm_gauss3()) {
-    if (b0)
-      hipLaunchKernelGGL((k_zgemm<true, true>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
-                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
-    else
-      hipLaunchKernelGGL((k_zgemm<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
-                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
-  } else {
-    if (b0)
-      hipLaunchKernelGGL((k_zgemm<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
-                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
-    else
-      hipLaunchKernelGGL((k_zgemm<false, false>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
-                         offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
-  }
+  if (beta.re == 0.0 && beta.im == 0.0)
+    hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA, rsA,
+                       csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  else
+    hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA,
+                       rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
 }
 
 // ---------------------------------------------------------------------------

@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void k_qr_simple(int n, cd* const* Hp, cd* con
 // ---------------------------------------------------------------------------
 #ifdef ORCWA_QR_PROFILE
 // per-phase shader-clock totals of matrix 0 (profiling builds only)
-__device__ unsigned long long g_qr_prof[16];
+__device__ unsigned long long g_qr_prof[24];
 #define QPROF_T0(v) unsigned long long v = (threadIdx.x == 0 && blockIdx.x == 0) ? clock64() : 0
 #define QPROF_ADD(slot, v) \
   if (threadIdx.x == 0 && blockIdx.x == 0) g_qr_prof[slot] += clock64() - (v)
@@ -1433,10 +1433,10 @@ __global__ void k_zero_below_subdiag(int n, cd* const* Ap) {
 
 #ifdef ORCWA_QR_PROFILE
 extern "C" int orcwa_prof_qr(double* out) {
-  unsigned long long h[16];
+  unsigned long long h[24];
   if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_qr_prof), sizeof(h)) != hipSuccess) return 1;
-  for (int i = 0; i < 16; ++i) out[i] = (double)h[i];
-  unsigned long long z[16] = {0};
+  for (int i = 0; i < 24; ++i) out[i] = (double)h[i];
+  unsigned long long z[24] = {0};
   if (hipMemcpyToSymbol(HIP_SYMBOL(g_qr_prof), z, sizeof(z)) != hipSuccess) return 1;
   return 0;
 }

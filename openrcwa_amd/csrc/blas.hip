@@ -18,67 +18,85 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // MFMA f64 16x16x4 lane layout (measured on gfx950): A[i][k] in lane i+16k,
 // B[k][j] in lane j+16k, D[i][j] in lane j+16(i%4), element i/4.
 // ---------------------------------------------------------------------------
-template <bool BETA0>
+// TM x TN output tile (TM, TN in {16, 32, 64}): 4 waves in WR x WC, each wave
+// (TM/WR) x (TN/WC) = BI x BJ MFMA 16 x 16 blocks.
+template <int TM, int TN>
+struct TileShape {
+  static constexpr int WR = (TM >= 32 && TN >= 32) ? 2 : (TN == 16 ? 4 : 1);
+  static constexpr int WC = 4 / WR;
+  static constexpr int BI = TM / (16 * WR), BJ = TN / (16 * WC);
+  static constexpr int LA = TM / 16, LB = TN / 16;   // loads per thread per K step of 16
+  static_assert(BI >= 1 && BJ >= 1 && WR * WC == 4, "tile shape");
+};
+
+template <bool BETA0, int TM, int TN>
 __global__ __launch_bounds__(256) void k_zgemm(
     int M, int N, int K, cd alpha, cd beta,
     const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
     const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
     cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
-  __shared__ double sAr[16][64];
-  __shared__ double sAi[16][64];
-  __shared__ double sBr[16][64];
-  __shared__ double sBi[16][64];
+  using TS = TileShape<TM, TN>;
+  constexpr int BI = TS::BI, BJ = TS::BJ, LA = TS::LA, LB = TS::LB, WC = TS::WC;
+  __shared__ double sAr[16][TM];
+  __shared__ double sAi[16][TM];
+  __shared__ double sBr[16][TN];
+  __shared__ double sBi[16][TN];
 
   const int b = blockIdx.z;
   const cd* __restrict__ A = Ap[b] + offA;
   const cd* __restrict__ B = Bp[b] + offB;
   cd* __restrict__ C = Cp[b] + offC;
-  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave / WC) * (16 * BI), wn = (wave % WC) * (16 * BJ);
   const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
 
-  // global loads run along whichever index of an operand has unit stride;
-  // each thread moves 4 elements of A and 4 of B per K step of 16
+  // global loads run along whichever index of an operand has unit stride
   const bool a_mfast = (rsA == 1), b_kfast = (rsB == 1);
-  int amm[4], akk[4], bkk[4], bnn[4];
-  const cd* pa[4];
-  const cd* pb[4];
-  bool aok[4], bok[4];
+  int amm[LA], akk[LA], bkk[LB], bnn[LB];
+  const cd* pa[LA];
+  const cd* pb[LB];
+  bool aok[LA], bok[LB];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < LA; ++j) {
     const int e = tid + 256 * j;
-    amm[j] = a_mfast ? (e & 63) : (e >> 4);
-    akk[j] = a_mfast ? (e >> 6) : (e & 15);
-    bkk[j] = b_kfast ? (e & 15) : (e >> 6);
-    bnn[j] = b_kfast ? (e >> 4) : (e & 63);
+    amm[j] = a_mfast ? (e % TM) : (e >> 4);
+    akk[j] = a_mfast ? (e / TM) : (e & 15);
     aok[j] = (m0 + amm[j] < M);
-    bok[j] = (n0 + bnn[j] < N);
     pa[j] = A + (long long)(m0 + amm[j]) * rsA + (long long)akk[j] * csA;
+  }
+#pragma unroll
+  for (int j = 0; j < LB; ++j) {
+    const int e = tid + 256 * j;
+    bkk[j] = b_kfast ? (e & 15) : (e / TN);
+    bnn[j] = b_kfast ? (e >> 4) : (e % TN);
+    bok[j] = (n0 + bnn[j] < N);
     pb[j] = B + (long long)bkk[j] * rsB + (long long)(n0 + bnn[j]) * csB;
   }
   const long long stepA = 16 * csA, stepB = 16 * rsB;
-  cd ra[4], rb[4];
+  cd ra[LA], rb[LB];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    ra[j] = (aok[j] && akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
-    rb[j] = (bok[j] && bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
-  }
+  for (int j = 0; j < LA; ++j) ra[j] = (aok[j] && akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
+#pragma unroll
+  for (int j = 0; j < LB; ++j) rb[j] = (bok[j] && bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
 
-  v4d accR[2][2], accI[2][2];
+  v4d accR[BI][BJ], accI[BI][BJ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < BJ; ++j) {
       accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
       accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
 
   for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < LA; ++j) {
       sAr[akk[j]][amm[j]] = ra[j].re;
       sAi[akk[j]][amm[j]] = sa * ra[j].im;
+    }
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
       sBr[bkk[j]][bnn[j]] = rb[j].re;
       sBi[bkk[j]][bnn[j]] = sb * rb[j].im;
     }
@@ -86,28 +104,34 @@ __global__ __launch_bounds__(256) void k_zgemm(
     // next K step in flight while this one runs on the MFMA units
     if (k0 + 16 < K) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < LA; ++j) {
         pa[j] += stepA;
-        pb[j] += stepB;
         ra[j] = (aok[j] && k0 + 16 + akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        pb[j] += stepB;
         rb[j] = (bok[j] && k0 + 16 + bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
       }
     }
 #pragma unroll
     for (int kq = 0; kq < 16; kq += 4) {
       const int ka = kq + (lane >> 4);
-      double ar[2], ai[2], br[2], bi[2];
+      double ar[BI], ai[BI], br[BJ], bi[BJ];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < BI; ++i) {
         ar[i] = sAr[ka][wm + 16 * i + (lane & 15)];
         ai[i] = sAi[ka][wm + 16 * i + (lane & 15)];
-        br[i] = sBr[ka][wn + 16 * i + (lane & 15)];
-        bi[i] = sBi[ka][wn + 16 * i + (lane & 15)];
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < BJ; ++j) {
+        br[j] = sBr[ka][wn + 16 * j + (lane & 15)];
+        bi[j] = sBi[ka][wn + 16 * j + (lane & 15)];
+      }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+      for (int i = 0; i < BI; ++i)
+#pragma unroll
+        for (int j = 0; j < BJ; ++j) {
           accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
           accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
           accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
@@ -118,21 +142,61 @@ __global__ __launch_bounds__(256) void k_zgemm(
   }
 
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gm = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
         const int gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N) {
-          const cd acc = mk(accR[i][j][r], accI[i][j][r]);
-          cd v = alpha * acc;
+          cd v = alpha * mk(accR[i][j][r], accI[i][j][r]);
           cd* p = C + gm * rsC + gn * csC;
           if (!BETA0) v += beta * (*p);
           *p = v;
         }
       }
+}
+
+template <int TM, int TN>
+static void zgemm_launch(hipStream_t s, int M, int N, int K, cd alpha, const cd* const* A,
+                         long long offA, long long rsA, long long csA, int conjA,
+                         const cd* const* B, long long offB, long long rsB, long long csB,
+                         int conjB, cd beta, cd* const* C, long long offC, long long rsC,
+                         long long csC, int batch) {
+  if (M <= 0 || N <= 0) return;
+  dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, batch);
+  if (beta.re == 0.0 && beta.im == 0.0)
+    hipLaunchKernelGGL((k_zgemm<true, TM, TN>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
+                       offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  else
+    hipLaunchKernelGGL((k_zgemm<false, TM, TN>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
+                       offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+}
+
+// rows of a strip narrower than 64: the smallest tile height that covers them
+template <int TN>
+static void zgemm_rows(hipStream_t s, int M, int N, int K, cd alpha, const cd* const* A,
+                       long long offA, long long rsA, long long csA, int conjA, const cd* const* B,
+                       long long offB, long long rsB, long long csB, int conjB, cd beta,
+                       cd* const* C, long long offC, long long rsC, long long csC, int batch) {
+  // (a 4-wave tile needs at least four 16 x 16 blocks)
+  if constexpr (TN == 64) {
+    if (M <= 16) {
+      zgemm_launch<16, 64>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
+                           beta, C, offC, rsC, csC, batch);
+      return;
+    }
+  }
+  if constexpr (TN >= 32) {
+    if (M <= 32) {
+      zgemm_launch<32, TN>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
+                           beta, C, offC, rsC, csC, batch);
+      return;
+    }
+  }
+  zgemm_launch<64, TN>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
+                       beta, C, offC, rsC, csC, batch);
 }
 
 void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
@@ -141,13 +205,28 @@ void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
                    cd beta, cd* const* C, long long offC, long long rsC, long long csC,
                    int batch) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
-  dim3 grid((M + 63) / 64, (N + 63) / 64, batch);
-  if (beta.re == 0.0 && beta.im == 0.0)
-    hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA, rsA,
-                       csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
-  else
-    hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA,
-                       rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  // 64 x 64 tiles over the largest multiple-of-64 block, narrower tiles on the
+  // remaining row and column strips (n = 450: 448 + a 2-wide strip, instead
+  // of a 64-wide tile row and column that are 97 % padding)
+  const int M0 = (M / 64) * 64, N0 = (N / 64) * 64, mr = M - M0, nr = N - N0;
+  if (M0 > 0 && N0 > 0)
+    zgemm_launch<64, 64>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
+                         beta, C, offC, rsC, csC, batch);
+  if (mr > 0 && N0 > 0)     // bottom strip
+    zgemm_rows<64>(s, mr, N0, K, alpha, A, offA + M0 * rsA, rsA, csA, conjA, B, offB, rsB, csB,
+                   conjB, beta, C, offC + M0 * rsC, rsC, csC, batch);
+  if (nr > 0) {             // right strip, all rows (corner included)
+    const long long oB = offB + N0 * csB, oC = offC + N0 * csC;
+    if (nr <= 16)
+      zgemm_rows<16>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
+                     oC, rsC, csC, batch);
+    else if (nr <= 32)
+      zgemm_rows<32>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
+                     oC, rsC, csC, batch);
+    else
+      zgemm_rows<64>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
+                     oC, rsC, csC, batch);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -17,7 +17,8 @@ def crand(*shape):
 
 
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (100, 70, 130),
-                                   (450, 450, 450)])
+                                   (450, 450, 450), (30, 200, 40), (130, 40, 70), (70, 20, 33),
+                                   (10, 100, 12), (20, 20, 20), (129, 250, 64)])
 def test_zgemm_shapes(gpu_ctx, M, N, K):
     A, B, C = crand(2, M, K), crand(2, K, N), crand(2, M, N)
     al, be = 0.7 - 0.2j, -1.1 + 0.4j

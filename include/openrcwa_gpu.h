@@ -122,6 +122,31 @@ int orcwa_plan_destroy(orcwa_plan *plan);
  * 5 eigensolver only.  Returns a negative value if unknown. */
 double orcwa_plan_phase_ms(orcwa_plan *plan, int phase);
 
+/* Slab scattering matrices in a reference basis, batched: for each instance b,
+ * the S-matrix of a slab of thickness wz[b] with modes (W, V, gamma) embedded
+ * in the basis (W0, V0): F0 = W^-1 W0, G0 = V^-1 V0, A = F0 + G0,
+ * B = F0 - G0, X = exp(i gamma wz lambda / 2 pi),
+ * Rud = (I - (A^-1 X B)^2)^-1 (A^-1 X B A^-1 X A - A^-1 B),
+ * Tuu = (I - (A^-1 X B)^2)^-1 (A^-1 X A - A^-1 X B A^-1 B); the slab is
+ * symmetric (Rdu = Rud, Tdd = Tuu).  Replaces
+ * dtmm::RCWAScatterMatrix::compute_v2 (rcwa/RCWAScatterMatrix.cpp:88-147).
+ * Matrices n x n column-major complex (re, im), batch-major; gamma [batch][n]
+ * complex; info [batch] (optional) gets ORCWA_STATUS_SINGULAR bits. */
+int orcwa_gpu_slab_smatrix(orcwa_gpu_context *ctx, int n, int batch, const double *lambda,
+                           const double *wz, const double *W, const double *V,
+                           const double *gamma, const double *W0, const double *V0,
+                           double *Rud, double *Tuu, int *info);
+
+/* Redheffer star product of S-matrices (rud, tuu, rdu, tdd), batched: `top`
+ * above `bottom`, each [batch][4][n][n] complex in that order; out = top * bottom:
+ *   C1 = (I - rdu_t Rud_b)^-1,  C2 = I + Rud_b C1 rdu_t,
+ *   Rdu = Rdu_b + Tdd_b rdu_t C2 Tuu_b,  Tdd = Tdd_b C1 tdd_t,
+ *   Rud = rud_t + tuu_t Rud_b C1 tdd_t,  Tuu = tuu_t C2 Tuu_b.
+ * The slab-on-top update of RCWAIntegrator::compute
+ * (core/RCWAIntegrator.cpp:57-67) is top = slab, bottom = accumulated stack. */
+int orcwa_gpu_redheffer_star(orcwa_gpu_context *ctx, int n, int batch, const double *top,
+                             const double *bottom, double *out, int *info);
+
 /* Layout-compatible records of the .ofd RCWA input globals
  * (include/orcwa_rcwa.h:27-35 and :45-52 of the reference): same field order
  * and types, so rcwa_run reads the arrays the reference's input module fills. */

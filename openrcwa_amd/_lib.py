@@ -41,7 +41,8 @@ class OrcwaResult(ctypes.Structure):
 # every symbol include/openrcwa_gpu.h declares
 EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_solve_batch",
             "orcwa_plan_create", "orcwa_plan_execute", "orcwa_plan_sync",
-            "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms", "rcwa_run"]
+            "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms",
+            "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "rcwa_run"]
 
 _lib = None
 

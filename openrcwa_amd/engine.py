@@ -172,3 +172,52 @@ def solve_batch(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0
         return p.results()
     finally:
         p.close()
+
+
+def _cplx(a, shape):
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.complex128), shape))
+    return a
+
+
+def slab_smatrix(ctx: Context, W, V, gamma, W0, V0, lam, wz):
+    """Slab S-matrices (Rud, Tuu) of a batch of slabs in reference bases
+    (orcwa_gpu_slab_smatrix; dtmm::RCWAScatterMatrix::compute_v2,
+    rcwa/RCWAScatterMatrix.cpp:88-147).  W, V, W0, V0: (B, n, n); gamma: (B, n);
+    lam, wz: (B,) or scalars.  Returns (Rud, Tuu, info)."""
+    lib = _lib.load()
+    W = np.asarray(W, dtype=np.complex128)
+    B, n, _ = W.shape
+    # column-major per matrix: store the transpose of each row-major matrix
+    cm = [np.ascontiguousarray(np.swapaxes(_cplx(x, (B, n, n)), 1, 2)) for x in (W, V, W0, V0)]
+    g = _cplx(gamma, (B, n))
+    lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, dtype=np.float64), (B,)))
+    wz = np.ascontiguousarray(np.broadcast_to(np.asarray(wz, dtype=np.float64), (B,)))
+    Rud = np.zeros((B, n, n), dtype=np.complex128)
+    Tuu = np.zeros((B, n, n), dtype=np.complex128)
+    info = np.zeros(B, dtype=np.int32)
+    _lib.check(lib.orcwa_gpu_slab_smatrix(
+        ctx.handle, n, B, _dptr(lam), _dptr(wz), _dptr(cm[0].view(np.float64)),
+        _dptr(cm[1].view(np.float64)), _dptr(g.view(np.float64)), _dptr(cm[2].view(np.float64)),
+        _dptr(cm[3].view(np.float64)), _dptr(Rud.view(np.float64)), _dptr(Tuu.view(np.float64)),
+        info.ctypes.data_as(POINTER(ctypes.c_int))), "orcwa_gpu_slab_smatrix")
+    return np.swapaxes(Rud, 1, 2), np.swapaxes(Tuu, 1, 2), info
+
+
+def redheffer_star(ctx: Context, top, bottom):
+    """Star product top * bottom of batches of S-matrices given as
+    (rud, tuu, rdu, tdd) arrays of shape (B, n, n) each (orcwa_gpu_redheffer_star;
+    core/RCWAIntegrator.cpp:57-67).  Returns ((rud, tuu, rdu, tdd), info)."""
+    lib = _lib.load()
+    t = np.stack([np.asarray(x, dtype=np.complex128) for x in top], axis=1)
+    b = np.stack([np.asarray(x, dtype=np.complex128) for x in bottom], axis=1)
+    B, _, n, _ = t.shape
+    tc = np.ascontiguousarray(np.swapaxes(t, 2, 3))
+    bc = np.ascontiguousarray(np.swapaxes(b, 2, 3))
+    out = np.zeros_like(tc)
+    info = np.zeros(B, dtype=np.int32)
+    _lib.check(lib.orcwa_gpu_redheffer_star(
+        ctx.handle, n, B, _dptr(tc.view(np.float64)), _dptr(bc.view(np.float64)),
+        _dptr(out.view(np.float64)), info.ctypes.data_as(POINTER(ctypes.c_int))),
+        "orcwa_gpu_redheffer_star")
+    o = np.swapaxes(out, 2, 3)
+    return (o[:, 0], o[:, 1], o[:, 2], o[:, 3]), info

@@ -487,19 +487,23 @@ def slab_scatter_matrix(W, V, gamma, W0, V0, wz, lam):
     return Rud, Tuu
 
 
-def redheffer_star(state, Rud, Tuu):
-    """One Redheffer star update of the accumulated (rud, tuu, rdu, tdd) with a
-    symmetric slab (Rud = Rdu, Tuu = Tdd) (core/RCWAIntegrator.cpp:57-67)."""
-    rud, tuu, rdu, tdd = state
+def redheffer_star_general(top, bottom):
+    """Redheffer star product top * bottom of S-matrices (rud, tuu, rdu, tdd),
+    in the form of core/RCWAIntegrator.cpp:57-67 (there the new slab is `top`
+    and the accumulated stack `bottom`)."""
+    rud, tuu, rdu, tdd = top
+    Rud, Tuu, Rdu, Tdd = bottom
     n = Rud.shape[0]
     I = np.eye(n)
     C1 = np.linalg.inv(I - rdu @ Rud)
     C2 = I + Rud @ C1 @ rdu
-    Rdu_new = Rud + Tuu @ rdu @ C2 @ Tuu
-    Tdd_new = Tuu @ C1 @ tdd
-    Rud_new = rud + tuu @ Rud @ C1 @ tdd
-    Tuu_new = tuu @ C2 @ Tuu
-    return Rud_new, Tuu_new, Rdu_new, Tdd_new
+    return (rud + tuu @ Rud @ C1 @ tdd, tuu @ C2 @ Tuu, Rdu + Tdd @ rdu @ C2 @ Tuu, Tdd @ C1 @ tdd)
+
+
+def redheffer_star(state, Rud, Tuu):
+    """The accumulated (rud, tuu, rdu, tdd) above a symmetric slab
+    (Rdu = Rud, Tdd = Tuu): redheffer_star_general(state, slab)."""
+    return redheffer_star_general(state, (Rud, Tuu, Rud, Tuu))
 
 
 # --------------------------------------------------------------------------

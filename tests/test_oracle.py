@@ -383,3 +383,26 @@ def test_redheffer_star_of_patterned_slabs_equals_rtcm():
     # (a dropped or transposed term is an O(1) difference)
     assert np.abs(st[0] - s.Rud).max() < 1e-10
     assert np.abs(st[3] - s.Tdd).max() < 1e-10
+
+
+def test_redheffer_star_identity_and_associativity():
+    # the star product has the identity (0, I, 0, I) on both sides and is
+    # associative; a dropped or reordered term breaks one of the two
+    rng = np.random.default_rng(4)
+    n = 7
+
+    def smat():
+        return tuple(0.3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+                     for _ in range(4))
+
+    eye, zero = np.eye(n, dtype=complex), np.zeros((n, n), complex)
+    E = (zero, eye, zero, eye)
+    a, b, c = smat(), smat(), smat()
+    for x, y in zip(o.redheffer_star_general(E, a), a):
+        assert np.abs(x - y).max() < 1e-13
+    for x, y in zip(o.redheffer_star_general(a, E), a):
+        assert np.abs(x - y).max() < 1e-13
+    left = o.redheffer_star_general(o.redheffer_star_general(a, b), c)
+    right = o.redheffer_star_general(a, o.redheffer_star_general(b, c))
+    for x, y in zip(left, right):
+        assert np.abs(x - y).max() < 1e-12

@@ -20,6 +20,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "layer-stack solves/sec at 2D 15x15 orders; max |dR,dT| vs FP64 oracle"
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and MFMA)
+# HBM bytes per 450 x 450 matrix of the eigensolver stage: 1.969 x FETCH_SIZE +
+# WRITE_SIZE over its dispatches (two rocprofv3 --pmc runs, the factor from the
+# plain copy kernel of the same run; profiles/pillar2d_eigensolver_pmc_traffic.md)
+EIG_TRAFFIC_PER_MATRIX_N450 = 1.713e9
 
 
 def eig_flops(n):
@@ -113,8 +117,10 @@ def main():
     eig_avg_s = eig_ms / args.steps / 1e3
     eig_alg = eig_flops(n) * B * n_pat
     achieved = eig_alg / eig_avg_s / 1e12
+    traffic = EIG_TRAFFIC_PER_MATRIX_N450 * B * n_pat if n == 450 else None
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "traffic_unit": "B per launch (eigensolver stage of one step)",
             "kernel": "batched zgeev (Hessenberg + Schur QR + eigenvectors), 100 n^3 flop/matrix",
             "launch_ms": eig_ms / args.steps}
 

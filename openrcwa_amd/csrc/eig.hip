@@ -735,19 +735,28 @@ __device__ __forceinline__ bool wave_small_qr(cd* B, cd* U, int m, bool wantu, i
   const double smlnum = kSafmin * ((double)m / kUlp);
   int hi = m - 1, its = 0, tot = 0;
   while (hi > 0) {
+    QPROF_T0(ts0);
     const bool flag = (lane >= 1 && lane <= hi) ? small_subdiag_lds(B, lane, 0, hi, smlnum) : false;
     const unsigned long long mask = __ballot(flag);
     const int l = mask ? 63 - __clzll(mask) : 0;
     wave_fence();
     if (l > 0 && lane == 0) B[l + (l - 1) * QLD] = mk(0.0, 0.0);
     wave_fence();
+    QPROF_ADD(19, ts0);
     if (l == hi) { --hi; its = 0; continue; }
     ++its;
     if (++tot > maxit) return false;
+    QPROF_T0(ts1);
+    QPROF_CNT(22);
     cd sh;
     if (its % 30 == 10) sh = mk(0.75 * cabs1(B[(l + 1) + l * QLD]), 0.0) + B[l + l * QLD];
     else if (its % 30 == 20) sh = mk(0.75 * cabs1(B[hi + (hi - 1) * QLD]), 0.0) + B[hi + hi * QLD];
     else sh = wilkinson_lds(B, hi);
+    QPROF_ADD(20, ts1);
+    QPROF_T0(ts2);
+#ifdef ORCWA_QR_PROFILE
+    if (threadIdx.x == 0 && blockIdx.x == 0) g_qr_prof[23] += hi - l;
+#endif
     for (int k = l; k < hi; ++k) {
       cd x0, x1;
       if (k == l) { x0 = B[l + l * QLD] - sh; x1 = B[(l + 1) + l * QLD]; }
@@ -782,6 +791,7 @@ __device__ __forceinline__ bool wave_small_qr(cd* B, cd* U, int m, bool wantu, i
       }
       wave_fence();
     }
+    QPROF_ADD(21, ts2);
   }
   return true;
 }
@@ -800,16 +810,22 @@ __device__ __forceinline__ bool lds_small_qr(QrShared& S, cd* B, int m, bool wan
 // X <- U^H X for rows [a, a+wa) of global X (ld), columns [c0, c1).  Each
 // wave owns 16-column blocks; the block's operand is loaded in two halves of
 // 32 rows whose loads are issued together, so two memory latencies are paid
-// per block.
+// per block.  Complex products in three real MFMAs (Gauss): with
+// conj(U) = a + ib and X = x + iy,
+//   P = a (x + y),  Q = (a + b) y,  R = (b - a) x,  re = P - Q,  im = P + R.
 __device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long long ld, int a, int wa, int c0,
                                  int c1) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int HALF = QW / 8;
   for (int j0 = c0 + 16 * wave; j0 < c1; j0 += 64) {
     const int j = j0 + (lane & 15);
-    v4d accR[4], accI[4];
+    v4d accP[4], accQ[4], accR[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) { accR[t] = (v4d){0, 0, 0, 0}; accI[t] = (v4d){0, 0, 0, 0}; }
+    for (int t = 0; t < 4; ++t) {
+      accP[t] = (v4d){0, 0, 0, 0};
+      accQ[t] = (v4d){0, 0, 0, 0};
+      accR[t] = (v4d){0, 0, 0, 0};
+    }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       cd cur[HALF];
@@ -821,14 +837,14 @@ __device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long 
 #pragma unroll
       for (int q = 0; q < HALF; ++q) {
         const int k = 4 * (HALF * h + q) + (lane >> 4);
-        const cd bv = cur[q];
+        const double x = cur[q].re, y = cur[q].im, xy = x + y;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // A[i][k] = conj(U[k][i])
-          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, bv.re, accR[t], 0, 0, 0);
-          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.im, bv.im, accR[t], 0, 0, 0);
-          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, bv.im, accI[t], 0, 0, 0);
-          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.im, bv.re, accI[t], 0, 0, 0);
+          const double ua = u.re, ub = -u.im;
+          accP[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, xy, accP[t], 0, 0, 0);
+          accQ[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua + ub, y, accQ[t], 0, 0, 0);
+          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ub - ua, x, accR[t], 0, 0, 0);
         }
       }
     }
@@ -837,22 +853,28 @@ __device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * t + (lane >> 4) + 4 * r;
-        if (i < wa && j < c1) X[(a + i) + (long long)j * ld] = mk(accR[t][r], accI[t][r]);
+        if (i < wa && j < c1)
+          X[(a + i) + (long long)j * ld] = mk(accP[t][r] - accQ[t][r], accP[t][r] + accR[t][r]);
       }
   }
 }
 
 // Y <- Y U for rows [r0, r1) of global Y (ld), columns [a, a+wa); same
-// blocking over 16-row blocks.
+// blocking over 16-row blocks and the same three-product form with
+// Y = a + ib, U = x + iy.
 __device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long long ld, int r0, int r1, int a,
                                 int wa) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int HALF = QW / 8;
   for (int i0 = r0 + 16 * wave; i0 < r1; i0 += 64) {
     const int i = i0 + (lane & 15);
-    v4d accR[4], accI[4];
+    v4d accP[4], accQ[4], accR[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) { accR[t] = (v4d){0, 0, 0, 0}; accI[t] = (v4d){0, 0, 0, 0}; }
+    for (int t = 0; t < 4; ++t) {
+      accP[t] = (v4d){0, 0, 0, 0};
+      accQ[t] = (v4d){0, 0, 0, 0};
+      accR[t] = (v4d){0, 0, 0, 0};
+    }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       cd cur[HALF];
@@ -864,14 +886,13 @@ __device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long l
 #pragma unroll
       for (int q = 0; q < HALF; ++q) {
         const int k = 4 * (HALF * h + q) + (lane >> 4);
-        const cd av = cur[q];
+        const double ya = cur[q].re, yb = cur[q].im, yab = ya + yb, yba = yb - ya;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // B[k][j] = U[k][j]
-          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, u.re, accR[t], 0, 0, 0);
-          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.im, u.im, accR[t], 0, 0, 0);
-          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, u.im, accI[t], 0, 0, 0);
-          accI[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.im, u.re, accI[t], 0, 0, 0);
+          accP[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, u.re + u.im, accP[t], 0, 0, 0);
+          accQ[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(yab, u.im, accQ[t], 0, 0, 0);
+          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(yba, u.re, accR[t], 0, 0, 0);
         }
       }
     }
@@ -881,7 +902,8 @@ __device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long l
       for (int r = 0; r < 4; ++r) {
         const int ii = i0 + (lane >> 4) + 4 * r;
         const int jj = 16 * t + (lane & 15);
-        if (ii < r1 && jj < wa) Y[ii + (long long)(a + jj) * ld] = mk(accR[t][r], accI[t][r]);
+        if (ii < r1 && jj < wa)
+          Y[ii + (long long)(a + jj) * ld] = mk(accP[t][r] - accQ[t][r], accP[t][r] + accR[t][r]);
       }
   }
 }

@@ -14,7 +14,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // GEMM: 64x64 output tile per workgroup (4 waves, 32x32 each = 2x2 MFMA
 // 16x16 sub-tiles), K staged through LDS 16 at a time, re/im split into
-// separate LDS planes so each MFMA reads one double per lane.
+// separate LDS planes so each MFMA reads one double per lane.  Complex
+// products in three real MFMAs (Gauss); the launch bound asks for two
+// workgroups per CU, which keeps the 64x64 tile's accumulators in VGPRs
+// without spilling.
 // MFMA f64 16x16x4 lane layout (measured on gfx950): A[i][k] in lane i+16k,
 // B[k][j] in lane j+16k, D[i][j] in lane j+16(i%4), element i/4.
 // ---------------------------------------------------------------------------
@@ -30,7 +33,7 @@ struct TileShape {
 };
 
 template <bool BETA0, int TM, int TN>
-__global__ __launch_bounds__(256) void k_zgemm(
+__global__ __launch_bounds__(256, 2) void k_zgemm(
     int M, int N, int K, cd alpha, cd beta,
     const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
     const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
@@ -80,13 +83,16 @@ __global__ __launch_bounds__(256) void k_zgemm(
 #pragma unroll
   for (int j = 0; j < LB; ++j) rb[j] = (bok[j] && bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
 
-  v4d accR[BI][BJ], accI[BI][BJ];
+  // Gauss form: T1 = ar br, T2 = ai bi, T3 = (ar + ai)(br + bi);
+  // re = T1 - T2, im = T3 - T1 - T2 (three real MFMAs per complex product)
+  v4d acc1[BI][BJ], acc2[BI][BJ], acc3[BI][BJ];
 #pragma unroll
   for (int i = 0; i < BI; ++i)
 #pragma unroll
     for (int j = 0; j < BJ; ++j) {
-      accR[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-      accI[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acc1[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acc2[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acc3[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
 
   for (int k0 = 0; k0 < K; k0 += 16) {
@@ -128,14 +134,18 @@ __global__ __launch_bounds__(256) void k_zgemm(
         br[j] = sBr[ka][wn + 16 * j + (lane & 15)];
         bi[j] = sBi[ka][wn + 16 * j + (lane & 15)];
       }
+      double as[BI], bs[BJ];
+#pragma unroll
+      for (int i = 0; i < BI; ++i) as[i] = ar[i] + ai[i];
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) bs[j] = br[j] + bi[j];
 #pragma unroll
       for (int i = 0; i < BI; ++i)
 #pragma unroll
         for (int j = 0; j < BJ; ++j) {
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accR[i][j], 0, 0, 0);
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], accR[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], accI[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], accI[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc1[i][j], 0, 0, 0);
+          acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc2[i][j], 0, 0, 0);
+          acc3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], acc3[i][j], 0, 0, 0);
         }
     }
     __syncthreads();
@@ -150,7 +160,8 @@ __global__ __launch_bounds__(256) void k_zgemm(
         const int gm = m0 + wm + 16 * i + (lane >> 4) + 4 * r;
         const int gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N) {
-          cd v = alpha * mk(accR[i][j][r], accI[i][j][r]);
+          const double t1 = acc1[i][j][r], t2 = acc2[i][j][r];
+          cd v = alpha * mk(t1 - t2, acc3[i][j][r] - t1 - t2);
           cd* p = C + gm * rsC + gn * csC;
           if (!BETA0) v += beta * (*p);
           *p = v;

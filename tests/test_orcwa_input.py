@@ -1,0 +1,302 @@
+"""The `.orcwa` parser and layer builder (openrcwa_amd/orcwa.py) on the CPU:
+rcwa/RCWAInput.cpp:12-453 and the layer construction of
+rcwa/RCWADriver.cpp:53-315.  Expected values are written out from the
+reference's semantics (units, defaults, shapes, the Lorentz/conductivity
+permittivity, the polarisation vectors), in the cases its own tests use
+(tests/test_rcwa_input.cpp)."""
+import math
+
+import numpy as np
+import pytest
+
+from openrcwa_amd import orcwa as oc
+
+HEAD = """OpenRCWA 4 2
+title = {title}
+xmesh = -2.5e-07 10 2.5e-07
+ymesh = -2.5e-07 10 2.5e-07
+"""
+
+
+def write(tmp_path, text, name="in.orcwa"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+FRESNEL = HEAD.format(title="fresnel air-glass") + """zmesh = -3e-07 10 0.0 10 7e-07
+material = 1 2.25 0 1 0
+geometry = 2 1 -2.5e-07 2.5e-07 -2.5e-07 2.5e-07 -3e-07 0
+planewave = 0 0 1
+pbc = 1 1 0
+rcwaorder = 6 0
+frequency1 = 4.0e14 7.5e14 14
+end
+"""
+
+
+def test_parse_units_defaults_and_lists(tmp_path):
+    p = oc.parse_rcwa_input(write(tmp_path, FRESNEL))
+    assert p.title == "fresnel air-glass"
+    assert (p.xmin, p.xmax) == pytest.approx((-0.25, 0.25), abs=1e-15)
+    assert (p.zmin, p.zmax) == pytest.approx((-0.3, 0.7), abs=1e-15)
+    assert p.Lx == pytest.approx(0.5) and p.Ly == pytest.approx(0.5)
+    # materials 0 (air) and 1 (PEC placeholder), then the file's material 2
+    assert p.material_eps == [1.0, 1.0 + 1e8j, 2.25]
+    assert p.material_mu == [1.0, 1.0, 1.0]
+    assert p.material_sigma == [0.0, 0.0, 0.0]
+    assert [d.active() for d in p.material_dispersion] == [False] * 3
+    assert len(p.boxes) == 1
+    b = p.boxes[0]
+    assert (b.material, b.shape) == (2, oc.SHAPE_BOX)
+    assert (b.z0, b.z1) == pytest.approx((-0.3, 0.0), abs=1e-15)
+    assert (p.theta, p.phi, p.pol, p.nHx, p.nHy) == (0.0, 0.0, 1, 6, 0)
+    # 15 frequencies, lambda = c / f in um, ascending
+    f = 4.0e14 + (7.5e14 - 4.0e14) * np.arange(15) / 14
+    assert np.allclose(p.lambdas, np.sort(2.99792458e8 / f * 1e6), rtol=1e-15)
+
+
+def test_frequency_lists(tmp_path):
+    base = HEAD.format(title="f") + "zmesh = -3e-07 10 0.0 10 7e-07\n"
+    p = oc.parse_rcwa_input(write(tmp_path, base + "frequency1 = 4.0e+14 5.0e+14 4\n"
+                                  "frequency2 = 6.0e+14 7.0e+14 4\nend\n"))
+    assert len(p.lambdas) == 10 and np.all(np.diff(p.lambdas) > 0)
+    # "frequency" is frequency1; a second frequency1 line is ignored
+    p = oc.parse_rcwa_input(write(tmp_path, base + "frequency = 4.0e+14 5.0e+14 4\n"
+                                  "frequency1 = 1e14 2e14 9\nend\n"))
+    assert len(p.lambdas) == 5
+    # wavelength: range, single value, duplicates merged
+    p = oc.parse_rcwa_input(write(tmp_path, base + "wavelength = 0.6 0.9 4\nend\n"))
+    assert np.allclose(p.lambdas, [0.6, 0.675, 0.75, 0.825, 0.9])
+    p = oc.parse_rcwa_input(write(tmp_path, base + "wavelength = 0.6\nlambda = 0.6 0.6 0\nend\n"))
+    assert p.lambdas == [0.6]
+
+
+def test_materials_dispersion_and_overrides(tmp_path, capsys):
+    text = HEAD.format(title="m") + """zmesh = -3e-07 10 0.0 10 7e-07
+material = 1 2.25 1.0e4 1 0
+material = 1 1.0 0 1 0
+material_dispersion = 3 1.0 1.37e16 2.73e13 0.0
+material_dispersion = 3 2.0 5e15 0 1e15
+material_dispersion = 3 4.0 0 1 1
+material_index = 2 1.5 0.1
+material_eps = 9 2 0
+material_mu = 2 1.2 0.1
+wavelength = 0.6
+end
+"""
+    p = oc.parse_rcwa_input(write(tmp_path, text))
+    assert p.material_sigma[2] == 1.0e4
+    assert p.material_eps[2] == pytest.approx((1.5 + 0.1j) ** 2)
+    assert p.material_mu[2] == 1.2 + 0.1j
+    d = p.material_dispersion[3]
+    # einf is the last line's value; a pole with ae = 0 is not added
+    assert d.einf == 4.0 and len(d.poles) == 2
+    assert (d.poles[0].ae, d.poles[0].be, d.poles[0].ce) == (1.37e16, 2.73e13, 0.0)
+    assert (d.poles[1].ae, d.poles[1].ce) == (5e15, 1e15)
+    assert "material 9 is not defined" in capsys.readouterr().err
+    assert oc.has_magnetic_material(p)
+
+
+def test_unknown_and_fdtd_keywords(tmp_path, capsys):
+    text = HEAD.format(title="k") + """zmesh = -3e-07 10 0.0 10 7e-07
+solver = 3000 100 1e-3
+abc = 1 5 5.0 1.1
+plot3dgeom = 1
+wavlength = 0.7
+rcwaordr = 3
+wavelength = 0.6
+end
+"""
+    p = oc.parse_rcwa_input(write(tmp_path, text))
+    assert p.lambdas == [0.6] and p.nHx == 10
+    err = capsys.readouterr().err
+    assert '"wavlength"' in err and '"rcwaordr"' in err
+    assert "solver" not in err and "plot3dgeom" not in err
+
+
+def test_parse_errors(tmp_path):
+    with pytest.raises(oc.OrcwaInputError, match="wavelength"):
+        oc.parse_rcwa_input(write(tmp_path, HEAD.format(title="x") + "planewave = 0 0 1\nend\n"))
+    with pytest.raises(oc.OrcwaInputError, match="OpenRCWA"):
+        oc.parse_rcwa_input(write(tmp_path, "NotOpenRCWA 4 2\nend\n"))
+    with pytest.raises(oc.OrcwaInputError, match="rcwaorder"):
+        oc.parse_rcwa_input(write(tmp_path, HEAD.format(title="x") + "rcwaorder = abc\n"
+                                  "wavelength = 0.6\nend\n"))
+    with pytest.raises(oc.OrcwaInputError, match="x period"):
+        oc.parse_rcwa_input(write(tmp_path, "OpenRCWA 4 2\nxmesh = 1e-7 1 1e-7\n"
+                                  "wavelength = 0.6\nend\n"))
+    with pytest.raises(oc.OrcwaInputError, match="cannot open"):
+        oc.parse_rcwa_input(str(tmp_path / "missing.orcwa"))
+    # numbers follow std::stod / std::stoi: a valid prefix is read
+    p = oc.parse_rcwa_input(write(tmp_path, HEAD.format(title="x") + "rcwaorder = 3.7 2x\n"
+                                  "wavelength = 0.6um\nend\n"))
+    assert (p.nHx, p.nHy, p.lambdas) == (3, 2, [0.6])
+
+
+def test_planewave_polarisation_fallback(tmp_path, capsys):
+    p = oc.parse_rcwa_input(write(tmp_path, HEAD.format(title="x") + "planewave = 10 20 7 30\n"
+                                  "wavelength = 0.6\nend\n"))
+    assert (p.theta, p.phi, p.pol, p.psi) == (10.0, 20.0, 1, 30.0)
+    assert "pol=7" in capsys.readouterr().err
+
+
+def test_shapes():
+    b = oc.RCWABox(2, oc.SHAPE_CYL_Z, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    assert b.contains_xy(0.5, 0.5) and b.contains_xy(1.0, 0.5)
+    assert not b.contains_xy(0.0, 0.0) and not b.contains_xy(1.0, 1.0)
+    b.shape = oc.SHAPE_BOX
+    assert b.contains_xy(0.0, 0.0) and not b.contains_xy(1.5, 0.5)
+    b.shape = oc.SHAPE_CYL_X
+    assert b.contains_xy(-5.0, 0.5) and not b.contains_xy(0.5, 1.5)
+    b.shape = oc.SHAPE_CYL_Y
+    assert b.contains_xy(0.5, -5.0) and not b.contains_xy(1.5, 0.5)
+    b.shape = oc.SHAPE_SPHERE
+    assert b.contains_xyz(0.5, 0.5, 0.5) and b.contains_xyz(0.5, 0.5, 0.99)
+    assert not b.contains_xyz(0.9, 0.5, 0.9) and not b.contains_xyz(0.5, 0.5, 1.5)
+    b.shape = oc.SHAPE_CYL_X
+    assert b.contains_xyz(0.5, 0.5, 0.5) and not b.contains_xyz(-5.0, 0.5, 0.5)
+    assert not b.contains_xyz(0.5, 0.9, 0.9) and b.contains_xyz(0.5, 0.5, 0.99)
+    b.shape = oc.SHAPE_CYL_Y
+    assert b.contains_xyz(0.5, 0.5, 0.5) and not b.contains_xyz(0.5, -5.0, 0.5)
+    assert not b.contains_xyz(0.9, 0.5, 0.9)
+    b.shape = 99
+    assert not b.contains_xy(0.5, 0.5) and not b.contains_xyz(0.5, 0.5, 0.5)
+
+
+def test_z_edges_and_grids():
+    p = oc.RCWAProblem(zmin=-1.0, zmax=2.0)
+    p.boxes = [oc.RCWABox(z0=0.0, z1=1.0), oc.RCWABox(z0=5.0, z1=6.0)]
+    assert oc.derive_z_edges(p) == [-1.0, 0.0, 1.0, 2.0]
+    # a sphere adds 7 inner slices of its z range
+    p = oc.RCWAProblem(xmin=-0.25, xmax=0.25, ymin=-0.25, ymax=0.25, zmin=-0.7, zmax=0.7)
+    p.boxes = [oc.RCWABox(2, oc.SHAPE_SPHERE, -0.2, 0.2, -0.2, 0.2, -0.2, 0.2)]
+    z = oc.derive_z_edges(p)
+    assert len(z) == 11 and np.allclose(np.diff(z[1:-1]), 0.05)
+    # curved x/y cross-sections: 8 staircase cells over the bounding box
+    xg = oc.common_x_grid(p)
+    assert len(xg) == 11 and np.allclose(np.diff(xg[1:-1]), 0.05)
+    # a box only contributes its clipped edges; CYL_X is flat along x
+    p.boxes = [oc.RCWABox(2, oc.SHAPE_BOX, -0.1, 0.4, -0.3, 0.1, 0, 1),
+               oc.RCWABox(2, oc.SHAPE_CYL_X, -0.2, 0.2, -0.2, 0.2, 0, 1)]
+    assert np.allclose(oc.common_x_grid(p), [-0.25, -0.2, -0.1, 0.2, 0.25])
+    yg = oc.common_y_grid(p)
+    assert np.allclose(yg, sorted({-0.25, 0.1, 0.25} | {-0.2 + 0.05 * i for i in range(9)}))
+
+
+def test_eps_at_dispersion_conductivity_and_order():
+    p = oc.RCWAProblem(background_eps=1.5 + 0.0j)
+    p.material_eps = [1.0, 1.0 + 1e8j, 2.25, 3.0]
+    p.material_sigma = [0.0, 0.0, 1.0e4, 0.0]
+    p.material_dispersion = [oc.MaterialDispersion() for _ in range(4)]
+    p.material_dispersion[3] = oc.MaterialDispersion(1.0, [oc.LorentzPole(1.37e16, 2.73e13, 0.0),
+                                                           oc.LorentzPole(2e15, 1e14, 3e15)])
+    p.boxes = [oc.RCWABox(2, oc.SHAPE_BOX, 0, 1, 0, 1, 0, 1),
+               oc.RCWABox(3, oc.SHAPE_BOX, 0.5, 1, 0, 1, 0, 1)]
+    lam = 0.6
+    w = 2 * math.pi * 2.99792458e14 / lam
+    assert oc.eps_at(p, 2.0, 0.5, 0.5, lam) == 1.5
+    assert oc.eps_at(p, 0.25, 0.5, 0.5, lam) == pytest.approx(2.25 + 1e4j / (w * 8.854187817e-12))
+    drude = 1.0 - 1.37e16 ** 2 / (w * w + 1j * 2.73e13 * w)
+    lor = 2e15 ** 2 / (3e15 ** 2 - w * w - 1j * 1e14 * w)
+    # the last box that contains the point wins
+    assert oc.eps_at(p, 0.75, 0.5, 0.5, lam) == pytest.approx(drude + lor, rel=1e-14)
+    assert drude.real < -10 and drude.imag > 0
+
+
+@pytest.mark.parametrize("pol,psi", [(1, 0), (2, 0), (3, 30), (4, 0), (5, 0)])
+@pytest.mark.parametrize("theta,phi", [(0.0, 0.0), (40.0, 30.0)])
+def test_incident_polarisation(pol, psi, theta, phi):
+    p = oc.RCWAProblem(theta=theta, phi=phi, pol=pol, psi=psi)
+    px, py = oc.incident_polarisation(p)
+    th, ph = math.radians(theta), math.radians(phi)
+    if theta == 0.0:
+        tm, te = np.array([1.0, 0.0]), np.array([0.0, 1.0])
+    else:
+        tm = np.array([math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph)])
+        te = np.array([-math.sin(ph), math.cos(ph)])
+    a = {1: (1, 0), 2: (0, 1), 3: (math.cos(math.radians(psi)), math.sin(math.radians(psi))),
+         4: (2 ** -0.5, 1j * 2 ** -0.5), 5: (2 ** -0.5, -1j * 2 ** -0.5)}[pol]
+    want = a[0] * tm + a[1] * te
+    assert np.allclose([px, py], want, atol=1e-15)
+    # unit 3D field: |a_TM|^2 + |a_TE|^2 = 1
+    assert abs(a[0]) ** 2 + abs(a[1]) ** 2 == pytest.approx(1.0)
+
+
+PILLAR = HEAD.format(title="pillar") + """zmesh = -6e-07 10 -2e-07 10 0.0 10 6e-07
+background = 0.5
+material = 1 2.25 0 1 0
+geometry = 2 1 -2.5e-07 2.5e-07 -2.5e-07 2.5e-07 -6e-07 -2e-07
+geometry = 2 11 -1.5e-07 1.5e-07 -1.5e-07 1.5e-07 -2e-07 0
+planewave = 30 20 4
+rcwaorder = 4 4
+wavelength = 0.6 0.8 2
+end
+"""
+
+
+def test_build_batch(tmp_path):
+    p = oc.parse_rcwa_input(write(tmp_path, PILLAR))
+    bt = oc.build_batch(p)
+    # slabs from the top: background (0.5) half-space, cylinder layer, substrate
+    assert np.allclose(bt["thickness"], [0.6, 0.2, 0.4])
+    assert np.allclose(bt["centres"], [0.3, -0.1, -0.4])
+    assert bt["eps"].shape == (3, 3, 10, 10)
+    assert np.all(bt["eps"][:, 0] == 0.5) and np.all(bt["eps"][:, 2] == 2.25)
+    mid = bt["eps"][0, 1]
+    assert set(np.unique(mid)) == {0.5, 2.25}
+    # staircase circle: the 8 x 8 cells of the bounding box, inside where the
+    # cell centre is within radius 0.15
+    c = 0.5 * (bt["xgrid"][1:] + bt["xgrid"][:-1])
+    inside = (c[None, :] ** 2 + c[:, None] ** 2) <= 0.15 ** 2
+    assert np.array_equal(mid == 2.25, inside)
+    # Bloch vector from the incidence medium's index, circular polarisation
+    n1 = math.sqrt(0.5)
+    k0 = 2 * math.pi / bt["lam"]
+    assert np.allclose(bt["kx0"], k0 * n1 * math.sin(math.radians(30)) * math.cos(math.radians(20)))
+    assert np.allclose(bt["ky0"], k0 * n1 * math.sin(math.radians(30)) * math.sin(math.radians(20)))
+    px, py = bt["pol"][0]
+    # transverse part of the unit field: |a_TM|^2 cos^2(theta) + |a_TE|^2
+    assert abs(px) ** 2 + abs(py) ** 2 == pytest.approx(0.5 * math.cos(math.radians(30)) ** 2 + 0.5)
+
+
+def test_build_batch_rejects(tmp_path):
+    mag = FRESNEL.replace("material = 1 2.25 0 1 0", "material = 1 2.25 0 1.5 0")
+    with pytest.raises(oc.OrcwaInputError, match="magnetic"):
+        oc.build_batch(oc.parse_rcwa_input(write(tmp_path, mag)))
+    patterned = FRESNEL.replace("-2.5e-07 2.5e-07 -2.5e-07 2.5e-07 -3e-07 0",
+                                "-1e-07 1e-07 -2.5e-07 2.5e-07 -3e-07 0")
+    with pytest.raises(oc.OrcwaInputError, match="homogeneous"):
+        oc.build_batch(oc.parse_rcwa_input(write(tmp_path, patterned)))
+
+
+def test_csv_format(tmp_path):
+    r = [oc.RCWAResult(0.59958492, 0.04, 0.96000001, 1e-9, np.array([0.04, 0.0]),
+                       np.array([0.5, 0.46000001])),
+         oc.RCWAResult(0.6, 0.0, 0.0, 1.0, np.zeros(0), np.zeros(0))]
+    path = tmp_path / "o.csv"
+    oc.write_csv(str(path), r[:1], False, 2)
+    assert path.read_text() == ("# lambda[um], R, T, A, R+T\n"
+                                "0.59958492, 0.04, 0.96000001, 1e-09, 1.0000000\n"
+                                .replace("1.0000000", oc._g8(0.04 + 0.96000001)))
+    oc.write_csv(str(path), r, True, 2)
+    lines = path.read_text().splitlines()
+    assert lines[0] == ("# lambda[um], R, T, A, R+T, R_order_0, R_order_1, T_order_0, "
+                        "T_order_1")
+    assert lines[1].endswith(", 0.04, 0, 0.5, 0.46000001")
+    # a grazing instance has no per-order columns, as the reference's empty REF
+    assert lines[2] == "0.6, 0, 0, 1, 0"
+
+
+def test_single_slab_is_two_copies(tmp_path):
+    text = HEAD.format(title="bg") + """zmesh = -3e-07 10 0.0 10 7e-07
+background = 2.25
+planewave = 0 0 1
+rcwaorder = 4 0
+frequency1 = 5.0e+14 5.0e+14 0
+end
+"""
+    bt = oc.build_batch(oc.parse_rcwa_input(write(tmp_path, text)))
+    assert bt["eps"].shape == (1, 2, 1, 1) and np.all(bt["eps"] == 2.25)
+    assert np.allclose(bt["thickness"], [1.0, 1.0]) and np.allclose(bt["centres"], [0.2, 0.2])
+

@@ -81,6 +81,12 @@ typedef struct {
   int n_pol;                 /* 1..8 incident polarisations */
   const double *pol;         /* [n_pol][4] = Re px, Im px, Re py, Im py: the transverse
                                 amplitudes of RCWASolver::generateHorizontalPlaneWave */
+  const double *mu;          /* NULL (mu = 1 everywhere) or the relative permeability,
+                                laid out as eps: Layer(coordX, coordY, eps, mu)
+                                (rcwa/Layer.h:29-37).  Layers with mu != 1 take the
+                                magnetic P and Q of rcwa/Layer.cpp:129-145, 150-164,
+                                206-213; the half-spaces must stay homogeneous in eps
+                                and mu. */
 } orcwa_instance;
 
 /* Output of one instance.  Orders k = (2Ny+1)(m+Nx) + (n+Ny), h = (2Nx+1)(2Ny+1)

@@ -246,6 +246,7 @@ extern "C" int rcwa_run(FILE* fp_log) {
     inst[f].eps = &eps[(size_t)f * L * ncell * 2];
     inst[f].n_pol = 2;
     inst[f].pol = pp;
+    inst[f].mu = nullptr;
     res[f].ref = &ref[(size_t)f * 2 * nOrd];
     res[f].trn = &trn[(size_t)f * 2 * nOrd];
     res[f].amp = nullptr;

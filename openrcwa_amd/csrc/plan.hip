@@ -47,18 +47,20 @@ struct Blob {
 struct StepGroup {
   int count = 0;                 // instances in the group
   // table offsets into the blob
-  size_t Wa = 0, Wb = 0, Vb = 0, binv_a = 0, E = 0, H = 0, LU = 0, inst = 0;
+  size_t Wa = 0, Va = 0, Wb = 0, Vb = 0, binv_a = 0, E = 0, H = 0, LU = 0, inst = 0;
   size_t pp = 0, M = 0;          // patterned a: its ProbParam and fe33^-1
 };
 
 struct StepTables {
-  StepGroup ua, pa;              // a homogeneous / a patterned
+  StepGroup ua, pa, pm;          // a homogeneous / patterned / patterned magnetic
   size_t keff_b = 0, k0d = 0;    // per instance of the chunk
   size_t keff_a = 0;             // per instance: keff of a patterned upper layer, else null
 };
 
 struct ChunkTables {
   int i0 = 0, ninst = 0, nprob = 0, nuni = 0;
+  int nprob_mag = 0;             // the last nprob_mag problems are magnetic
+  size_t mag_mats = 0;           // MagMats of the magnetic problems
   size_t prob_params = 0, inst_of_prob = 0, uni_params = 0, rt_params = 0;
   // patterned problem slots
   size_t pA = 0, pAorig = 0, pW = 0, pV = 0, pZ = 0, pX = 0, pQW = 0;
@@ -84,8 +86,10 @@ struct orcwa_plan {
   std::vector<int> npol;
   std::vector<cd> pol;             // [count][8][2]
   std::vector<int> gid;            // [count*L]: grid index, -1 homogeneous
+  std::vector<int> mgid;           // [count*L]: permeability grid of a magnetic layer, else -1
   std::vector<cd> ueps;            // [count*L]: value of a homogeneous layer
-  std::vector<cd> eps_ref, eps_trn;
+  std::vector<cd> umu;             // [count*L]: its permeability
+  std::vector<cd> eps_ref, eps_trn, mu_ref, mu_trn;
   int G = 0;
   std::vector<cd> grid_vals;       // [G*ncell]
   std::vector<int> prob_inst;      // global problem -> instance
@@ -256,7 +260,10 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   p->thick.assign(st->thickness, st->thickness + p->L);
   const int L = p->L, h = p->h, n = p->n, ncell = p->ncell;
 
-  // ---- instances: classify layers, dedupe grids
+  // ---- instances: classify layers, dedupe grids.  A layer is homogeneous
+  // when eps and mu are uniform, a (non-magnetic) problem when eps is
+  // patterned and mu == 1, and a magnetic problem otherwise
+  // (rcwa/Layer.cpp:66-72, 129-145); eps and mu grids share one list.
   std::map<std::vector<double>, int> grid_index;
   p->lam.resize(count);
   p->kx0.resize(count);
@@ -264,9 +271,32 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   p->npol.resize(count);
   p->pol.assign((size_t)count * kMaxPol * 2, mk(0, 0));
   p->gid.assign((size_t)count * L, -1);
+  p->mgid.assign((size_t)count * L, -1);
   p->ueps.assign((size_t)count * L, mk(0, 0));
+  p->umu.assign((size_t)count * L, mk(1, 0));
   p->eps_ref.resize(count);
   p->eps_trn.resize(count);
+  p->mu_ref.resize(count);
+  p->mu_trn.resize(count);
+  auto grid_of = [&](const double* v) {
+    std::vector<double> key(v, v + 2 * ncell);
+    auto it = grid_index.find(key);
+    if (it != grid_index.end()) return it->second;
+    const int g = (int)grid_index.size();
+    grid_index.emplace(std::move(key), g);
+    for (int c = 0; c < ncell; ++c) p->grid_vals.push_back(mk(v[2 * c], v[2 * c + 1]));
+    return g;
+  };
+  auto uniform_of = [&](const double* v) {
+    for (int c = 1; c < ncell; ++c)
+      if (v[2 * c] != v[0] || v[2 * c + 1] != v[1]) return false;
+    return true;
+  };
+  auto has_zero = [&](const double* v) {
+    for (int c = 0; c < ncell; ++c)
+      if (v[2 * c] == 0.0 && v[2 * c + 1] == 0.0) return true;
+    return false;
+  };
   for (int i = 0; i < count; ++i) {
     const orcwa_instance& I = inst[i];
     if (!(I.lambda > 0) || !std::isfinite(I.lambda)) {
@@ -288,32 +318,19 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
     }
     for (int l = 0; l < L; ++l) {
       const double* e = I.eps + (size_t)l * ncell * 2;
-      bool uniform = true;
-      for (int c = 1; c < ncell && uniform; ++c)
-        if (e[2 * c] != e[0] || e[2 * c + 1] != e[1]) uniform = false;
-      if (uniform) {
-        p->ueps[(size_t)i * L + l] = mk(e[0], e[1]);
-        if (e[0] == 0.0 && e[1] == 0.0) {
-          delete p;
-          return fail("zero permittivity");
-        }
+      const double* m = I.mu ? I.mu + (size_t)l * ncell * 2 : nullptr;
+      if (has_zero(e) || (m && has_zero(m))) {
+        delete p;
+        return fail("zero permittivity or permeability");
+      }
+      const bool mu_one = !m || (uniform_of(m) && m[0] == 1.0 && m[1] == 0.0);
+      const size_t il = (size_t)i * L + l;
+      if (uniform_of(e) && (!m || uniform_of(m))) {
+        p->ueps[il] = mk(e[0], e[1]);
+        if (m) p->umu[il] = mk(m[0], m[1]);
       } else {
-        for (int c = 0; c < ncell; ++c)
-          if (e[2 * c] == 0.0 && e[2 * c + 1] == 0.0) {
-            delete p;
-            return fail("zero permittivity");
-          }
-        std::vector<double> key(e, e + 2 * ncell);
-        auto it = grid_index.find(key);
-        int g;
-        if (it == grid_index.end()) {
-          g = (int)grid_index.size();
-          grid_index.emplace(std::move(key), g);
-          for (int c = 0; c < ncell; ++c) p->grid_vals.push_back(mk(e[2 * c], e[2 * c + 1]));
-        } else {
-          g = it->second;
-        }
-        p->gid[(size_t)i * L + l] = g;
+        p->gid[il] = grid_of(e);
+        if (!mu_one) p->mgid[il] = grid_of(m);
       }
     }
     if (p->gid[(size_t)i * L] >= 0 || p->gid[(size_t)i * L + L - 1] >= 0) {
@@ -325,6 +342,8 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
     }
     p->eps_ref[i] = p->ueps[(size_t)i * L];
     p->eps_trn[i] = p->ueps[(size_t)i * L + L - 1];
+    p->mu_ref[i] = p->umu[(size_t)i * L];
+    p->mu_trn[i] = p->umu[(size_t)i * L + L - 1];
   }
   p->G = (int)grid_index.size();
 
@@ -420,10 +439,7 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   p->nprob_total = 0;
   for (int i = 0; i < count; ++i)
     for (int l = 0; l < L; ++l)
-      if (p->gid[(size_t)i * L + l] >= 0) {
-        p->prob_inst.push_back(i);
-        ++p->nprob_total;
-      }
+      if (p->gid[(size_t)i * L + l] >= 0) ++p->nprob_total;
   const size_t o_pinfo = take((size_t)std::max<long long>(1, p->nprob_total) * sizeof(int));
   const size_t o_resid = take((size_t)std::max<long long>(1, p->nprob_total) * sizeof(double));
   const size_t o_ref = take((size_t)count * p->maxpol * h * sizeof(double));
@@ -525,36 +541,52 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
     std::vector<ProbParam> pp;
     std::vector<int> inst_of;
     std::vector<UniParam> up;
-    for (int ii = 0; ii < ct.ninst; ++ii) {
-      const int i = ct.i0 + ii;
-      const double inv_k0 = p->lam[i] / (2.0 * kPi);
-      for (int l = 0; l < L; ++l) {
-        const int g = p->gid[(size_t)i * L + l];
-        if (g >= 0) {
-          pidx[(size_t)ii * L + l] = (int)pp.size();
-          ProbParam q;
-          q.kx0 = p->kx0[i];
-          q.ky0 = p->ky0[i];
-          q.inv_k0 = inv_k0;
-          q.grid = g;
-          q.pad = 0;
-          pp.push_back(q);
-          inst_of.push_back(i);
-        } else {
-          uidx[(size_t)ii * L + l] = (int)up.size();
-          UniParam u;
-          u.eps = p->ueps[(size_t)i * L + l];
-          u.kx0 = p->kx0[i];
-          u.ky0 = p->ky0[i];
-          u.inv_k0 = inv_k0;
-          u.inst = i;
-          u.pad = 0;
-          up.push_back(u);
+    // non-magnetic problems first, then the magnetic ones (each group is a
+    // contiguous range of the problem tables)
+    for (int pass = 0; pass < 2; ++pass)
+      for (int ii = 0; ii < ct.ninst; ++ii) {
+        const int i = ct.i0 + ii;
+        const double inv_k0 = p->lam[i] / (2.0 * kPi);
+        for (int l = 0; l < L; ++l) {
+          const size_t il = (size_t)i * L + l;
+          const int g = p->gid[il];
+          if (g >= 0) {
+            if ((p->mgid[il] >= 0) != (pass == 1)) continue;
+            pidx[(size_t)ii * L + l] = (int)pp.size();
+            ProbParam q;
+            q.kx0 = p->kx0[i];
+            q.ky0 = p->ky0[i];
+            q.inv_k0 = inv_k0;
+            q.grid = g;
+            q.mgrid = p->mgid[il];
+            pp.push_back(q);
+            inst_of.push_back(i);
+            p->prob_inst.push_back(i);
+            if (pass == 1) ++ct.nprob_mag;
+          } else if (pass == 0) {
+            uidx[(size_t)ii * L + l] = (int)up.size();
+            UniParam u;
+            u.eps = p->ueps[il];
+            u.mu = p->umu[il];
+            u.kx0 = p->kx0[i];
+            u.ky0 = p->ky0[i];
+            u.inv_k0 = inv_k0;
+            u.inst = i;
+            u.pad = 0;
+            up.push_back(u);
+          }
         }
       }
-    }
     ct.nprob = (int)pp.size();
     ct.nuni = (int)up.size();
+    {
+      std::vector<MagMats> mm;
+      for (const ProbParam& q : pp)
+        if (q.mgrid >= 0)
+          mm.push_back({p->gE11[q.grid], p->gE22[q.grid], p->gM[q.grid], p->gE11[q.mgrid],
+                        p->gE22[q.mgrid], p->gM[q.mgrid]});
+      ct.mag_mats = B.put(mm);
+    }
     prob_global += ct.nprob;
     ct.prob_params = B.put(pp);
     ct.inst_of_prob = B.put(inst_of);
@@ -629,14 +661,25 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
       std::vector<double> k0d;
       std::vector<cd*> uaWb, uaVb, uaBinv, uaE, uaH;
       std::vector<cd*> paWa, paWb, paVb, paE, paH, paLU, paM, ka;
+      std::vector<cd*> pmWa, pmVa, pmWb, pmVb, pmE, pmH, pmLU;
       std::vector<ProbParam> paPP;
-      std::vector<int> uaInst, paInst;
+      std::vector<int> uaInst, paInst, pmInst;
       for (int ii = 0; ii < ct.ninst; ++ii) {
         const int i = ct.i0 + ii;
         kb.push_back(Kof(ii, b));
         k0d.push_back(2.0 * kPi / p->lam[i] * p->thick[b]);
-        ka.push_back(pidx[(size_t)ii * L + a] >= 0 ? Kof(ii, a) : nullptr);
-        if (pidx[(size_t)ii * L + a] < 0) {
+        const bool a_mag = pidx[(size_t)ii * L + a] >= 0 && p->mgid[(size_t)i * L + a] >= 0;
+        ka.push_back(pidx[(size_t)ii * L + a] >= 0 && !a_mag ? Kof(ii, a) : nullptr);
+        if (a_mag) {
+          pmWa.push_back(Wof(ii, a));
+          pmVa.push_back(Vof(ii, a));
+          pmWb.push_back(Wof(ii, b));
+          pmVb.push_back(Vof(ii, b));
+          pmE.push_back(islot(ii, 2));
+          pmH.push_back(islot(ii, 3));
+          pmLU.push_back(islot(ii, 4 + sidx));
+          pmInst.push_back(i);
+        } else if (pidx[(size_t)ii * L + a] < 0) {
           uaWb.push_back(Wof(ii, b));
           uaVb.push_back(Vof(ii, b));
           uaBinv.push_back(Bof(ii, a));
@@ -676,6 +719,15 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
       stt.pa.inst = B.put(paInst);
       stt.pa.pp = B.put(paPP);
       stt.pa.M = B.put(paM);
+      stt.pm.count = (int)pmInst.size();
+      stt.pm.Wa = B.put(pmWa);
+      stt.pm.Va = B.put(pmVa);
+      stt.pm.Wb = B.put(pmWb);
+      stt.pm.Vb = B.put(pmVb);
+      stt.pm.E = B.put(pmE);
+      stt.pm.H = B.put(pmH);
+      stt.pm.LU = B.put(pmLU);
+      stt.pm.inst = B.put(pmInst);
       ct.steps.push_back(stt);
     }
     std::vector<RtParam> rp(ct.ninst);
@@ -685,6 +737,8 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
       std::memset(&r, 0, sizeof(r));
       r.eps_ref = p->eps_ref[i];
       r.eps_trn = p->eps_trn[i];
+      r.mu_ref = p->mu_ref[i];
+      r.mu_trn = p->mu_trn[i];
       r.kx0 = p->kx0[i];
       r.ky0 = p->ky0[i];
       r.inv_k0 = p->lam[i] / (2.0 * kPi);
@@ -772,9 +826,20 @@ int orcwa_plan_execute(orcwa_plan* p) {
       cd* const* A = p->dev<cd* const*>(ct.pA);
       cd* const* Wp = p->dev<cd* const*>(ct.pW);
       const cd* const* Wc = p->dev<const cd* const*>(ct.pW);
+      const int nm = ct.nprob_mag, nq = np - nm;   // magnetic problems at [nq, np)
       launch_build_pq(s, g, p->dev<const ProbParam*>(ct.prob_params),
                       p->dev<const GridMats*>(p->t_gridmats), A, p->dev<cd* const*>(ct.pAorig),
-                      np);
+                      nq);
+      if (nm > 0) {
+        // P in the V slot and Q in the QW slot, PQ = P Q (rcwa/WaveEquationSolver.cpp:30)
+        cd* const* Pm = p->dev<cd* const*>(ct.pV) + nq;
+        cd* const* Qm = p->dev<cd* const*>(ct.pQW) + nq;
+        launch_build_pq_mag(s, g, p->dev<const ProbParam*>(ct.prob_params) + nq,
+                            p->dev<const MagMats*>(ct.mag_mats), Pm, Qm, nm);
+        zgemm_batched(s, n, n, n, one, (const cd* const*)Pm, 0, 1, n, 0, (const cd* const*)Qm, 0,
+                      1, n, 0, zero, A + nq, 0, 1, n, nm);
+        copy_batched(s, nn, (const cd* const*)(A + nq), p->dev<cd* const*>(ct.pAorig) + nq, nm);
+      }
       EigWorkspace ws;
       ws.Z = p->dev<cd**>(ct.pZ);
       ws.X = p->dev<cd**>(ct.pX);
@@ -786,13 +851,25 @@ int orcwa_plan_execute(orcwa_plan* p) {
       HIPCHK(hipEventRecord(p->ev[6], s));
       // QW1 = [E22 W_bottom ; E11 W_top]
       cd* const* QW = p->dev<cd* const*>(ct.pQW);
-      zgemm_batched(s, h, n, h, one, p->dev<const cd* const*>(ct.pE22), 0, 1, h, 0, Wc, h, 1, n, 0,
-                    zero, QW, 0, 1, n, np);
-      zgemm_batched(s, h, n, h, one, p->dev<const cd* const*>(ct.pE11), 0, 1, h, 0, Wc, 0, 1, n, 0,
-                    zero, QW, h, 1, n, np);
-      launch_build_v(s, g, p->dev<const ProbParam*>(ct.prob_params), Wc,
-                     p->dev<const cd* const*>(ct.pQW), p->d_lam, p->d_keffp,
-                     p->dev<cd* const*>(ct.pV), p->d_info, p->dev<const int*>(ct.inst_of_prob), np);
+      if (nq > 0) {
+        zgemm_batched(s, h, n, h, one, p->dev<const cd* const*>(ct.pE22), 0, 1, h, 0, Wc, h, 1, n,
+                      0, zero, QW, 0, 1, n, nq);
+        zgemm_batched(s, h, n, h, one, p->dev<const cd* const*>(ct.pE11), 0, 1, h, 0, Wc, 0, 1, n,
+                      0, zero, QW, h, 1, n, nq);
+        launch_build_v(s, g, p->dev<const ProbParam*>(ct.prob_params), Wc,
+                       p->dev<const cd* const*>(ct.pQW), p->d_lam, p->d_keffp,
+                       p->dev<cd* const*>(ct.pV), p->d_info, p->dev<const int*>(ct.inst_of_prob),
+                       nq);
+      }
+      if (nm > 0) {
+        // Q W of the magnetic problems into the Z slot, then V = Q W diag(1/keff)
+        cd* const* Zm = p->dev<cd* const*>(ct.pZ) + nq;
+        zgemm_batched(s, n, n, n, one, p->dev<const cd* const*>(ct.pQW) + nq, 0, 1, n, 0, Wc + nq,
+                      0, 1, n, 0, zero, Zm, 0, 1, n, nm);
+        launch_build_v_mag(s, n, (const cd* const*)Zm, p->d_lam + (size_t)nq * n,
+                           p->d_keffp + (size_t)nq * n, p->dev<cd* const*>(ct.pV) + nq, p->d_info,
+                           p->dev<const int*>(ct.inst_of_prob) + nq, nm);
+      }
     }
     if (nu > 0)
       launch_uniform(s, g, p->dev<const UniParam*>(ct.uni_params), p->d_keffu,
@@ -833,6 +910,20 @@ int orcwa_plan_execute(orcwa_plan* p) {
         zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
         zgetrs_batched(s, n, 2 * n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.E), q.count,
                        p->d_perm);
+      }
+      if (st.pm.count > 0) {
+        // a magnetic patterned upper layer: E = W_a^-1 W_b and H = V_a^-1 V_b,
+        // one LU each (rcwa/RCWASolver.cpp:537-543)
+        const StepGroup& q = st.pm;
+        cd* const* qLU = p->dev<cd* const*>(q.LU);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Wa), qLU, q.count);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Wb), p->dev<cd* const*>(q.E), q.count);
+        zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
+        zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.E), q.count, p->d_perm);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Va), qLU, q.count);
+        copy_batched(s, nn, p->dev<const cd* const*>(q.Vb), p->dev<cd* const*>(q.H), q.count);
+        zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
+        zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.H), q.count, p->d_perm);
       }
       // A = E + H (in E), B = E - H (in H)
       launch_sum_diff(s, n, E, H, p->dev<const cd* const*>(st.keff_a), ni);

@@ -38,13 +38,25 @@ struct GridMats {
 struct ProbParam {
   double kx0, ky0;     // Bloch offsets (1/length)
   double inv_k0;       // 1/k0 (length)
-  int grid;            // grid index
-  int pad;
+  int grid;            // grid index (permittivity)
+  int mgrid;           // grid index of the permeability (magnetic problems), else -1
+};
+
+// convolution matrices of a magnetic problem: fe11, fe22, fe33^-1 of the
+// permittivity grid and fmu11, fmu22, fmu33^-1 of the permeability grid
+struct MagMats {
+  const cd* E11;
+  const cd* E22;
+  const cd* Ei;
+  const cd* M11;
+  const cd* M22;
+  const cd* Mi;
 };
 
 // one homogeneous-layer problem
 struct UniParam {
   cd eps;
+  cd mu;
   double kx0, ky0, inv_k0;
   int inst;
   int pad;
@@ -53,6 +65,7 @@ struct UniParam {
 // plane-wave response of one instance
 struct RtParam {
   cd eps_ref, eps_trn;
+  cd mu_ref, mu_trn;
   double kx0, ky0, inv_k0;
   int npol;
   int pad;
@@ -72,6 +85,10 @@ void launch_build_pq(hipStream_t s, const ConvGeom& g, const ProbParam* params, 
 void launch_build_v(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* W,
                     const cd* const* QW1, const cd* lam, cd* keff, cd* const* V, int* info,
                     const int* inst_of, int nprob);
+void launch_build_pq_mag(hipStream_t s, const ConvGeom& g, const ProbParam* params,
+                         const MagMats* mm, cd* const* P, cd* const* Q, int nprob);
+void launch_build_v_mag(hipStream_t s, int n, const cd* const* QW, const cd* lam, cd* keff,
+                        cd* const* V, int* info, const int* inst_of, int nprob);
 void launch_zero_mats(hipStream_t s, long long count, cd* const* A, int batch);
 void launch_uniform(hipStream_t s, const ConvGeom& g, const UniParam* params, cd* keff,
                     cd* const* V, cd* binv, int* info, int nuni);

@@ -283,6 +283,57 @@ __global__ void k_build_v(const ConvGeom g, const ProbParam* params, const cd* c
   V[pi][e] = (kf.re == 0.0 && kf.im == 0.0) ? v : cdiv(v, kf);
 }
 
+// P and Q of a magnetic patterned layer (rcwa/Layer.cpp:129-145, 150-164,
+// 206-213) in k0 units, with F1h = fe33^-1 Kx, F2h = fe33^-1 Ky,
+// G1h = fmu33^-1 Kx, G2h = fmu33^-1 Ky (Kx, Ky diagonal):
+//   P = [[Kx F2h, fmu22 - Kx F1h], [Ky F2h - fmu11, -Ky F1h]]
+//   Q = [[-Kx G2h, Kx G1h - fe22], [fe11 - Ky G2h, Ky G1h]]
+__global__ void k_build_pq_mag(const ConvGeom g, const ProbParam* params, const MagMats* mm,
+                               cd* const* Pp, cd* const* Qp) {
+  const int pi = blockIdx.y;
+  const ProbParam p = params[pi];
+  const MagMats M = mm[pi];
+  const int h = g.nx * g.ny, n = 2 * h;
+  const long long hh = (long long)h * h;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= hh) return;
+  const int r = (int)(e % h), c = (int)(e / h);
+  double kxr, kyr, kxc, kyc;
+  k_vectors(g, p, r, kxr, kyr);
+  k_vectors(g, p, c, kxc, kyc);
+  const cd ei = M.Ei[e], mi = M.Mi[e];
+  cd* P = Pp[pi];
+  cd* Q = Qp[pi];
+  const long long i11 = r + (long long)c * n, i21 = (h + r) + (long long)c * n;
+  const long long i12 = r + (long long)(h + c) * n, i22 = (h + r) + (long long)(h + c) * n;
+  P[i11] = (kxr * kyc) * ei;
+  P[i12] = M.M22[e] - (kxr * kxc) * ei;
+  P[i21] = (kyr * kyc) * ei - M.M11[e];
+  P[i22] = -((kyr * kxc) * ei);
+  Q[i11] = -((kxr * kyc) * mi);
+  Q[i12] = (kxr * kxc) * mi - M.E22[e];
+  Q[i21] = M.E11[e] - (kyr * kyc) * mi;
+  Q[i22] = (kyr * kxc) * mi;
+}
+
+// keff = GammaSqrtOp(lambda), V[:, c] = (Q W)[:, c] / keff_c
+// (rcwa/WaveEquationSolver.cpp:32-36) from the product Q W of a magnetic layer
+__global__ void k_build_v_mag(int n, const cd* const* QW, const cd* lam, cd* keff_all,
+                              cd* const* V, int* info, const int* inst_of) {
+  const int pi = blockIdx.y;
+  const long long nn = (long long)n * n;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  const cd kf = gamma_sqrt(lam[(long long)pi * n + c]);
+  if (r == 0) {
+    keff_all[(long long)pi * n + c] = kf;
+    if (kf.re == 0.0 && kf.im == 0.0) atomicOr(&info[inst_of[pi]], ORCWA_STATUS_SINGULAR);
+  }
+  const cd v = QW[pi][e];
+  V[pi][e] = (kf.re == 0.0 && kf.im == 0.0) ? v : cdiv(v, kf);
+}
+
 __global__ void k_zero_mats(long long count, cd* const* A) {
   cd* a = A[blockIdx.y];
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
@@ -290,9 +341,10 @@ __global__ void k_zero_mats(long long count, cd* const* A) {
     a[e] = mk(0.0, 0.0);
 }
 
-// Homogeneous layer (eps, mu = 1): W = I, keff = GammaSqrtOp(eps - kx^2 - ky^2)
-// for both field components of every order, V = Q diag(1/keff) which is
-// 2x2-block diagonal per order.  Writes keff (n), the non-zero entries of the
+// Homogeneous layer (eps, mu): W = I, keff = GammaSqrtOp(eps mu - kx^2 - ky^2)
+// for both field components of every order (P Q = (eps mu - kx^2 - ky^2) I),
+// V = Q diag(1/keff) with Q = [[-kx ky/mu, kx^2/mu - eps], [eps - ky^2/mu,
+// kx ky/mu]] per order, 2x2-block diagonal.  Writes keff (n), the non-zero entries of the
 // dense V (zeroed beforehand by k_zero_mats) and the inverse 2x2 blocks
 // (4h: b00, b01, b10, b11 of V^{-1}).
 __global__ void k_uniform(const ConvGeom g, const UniParam* params, cd* keff_all,
@@ -309,7 +361,7 @@ __global__ void k_uniform(const ConvGeom g, const UniParam* params, cd* keff_all
   p.kx0 = u.kx0; p.ky0 = u.ky0; p.inv_k0 = u.inv_k0; p.grid = 0;
   double kx, ky;
   k_vectors(g, p, k, kx, ky);
-  const cd kz2 = u.eps - mk(kx * kx + ky * ky, 0.0);
+  const cd kz2 = u.eps * u.mu - mk(kx * kx + ky * ky, 0.0);
   const cd kf = gamma_sqrt(kz2);
   keff_all[(long long)ui * n + k] = kf;
   keff_all[(long long)ui * n + h + k] = kf;
@@ -318,10 +370,11 @@ __global__ void k_uniform(const ConvGeom g, const UniParam* params, cd* keff_all
     return;
   }
   const cd ik = cinv(kf);
-  const cd v00 = (-(kx * ky)) * ik;
-  const cd v01 = (mk(kx * kx, 0.0) - u.eps) * ik;
-  const cd v10 = (u.eps - mk(ky * ky, 0.0)) * ik;
-  const cd v11 = (ky * kx) * ik;
+  const cd imu = cinv(u.mu);
+  const cd v00 = (-(kx * ky)) * imu * ik;
+  const cd v01 = ((kx * kx) * imu - u.eps) * ik;
+  const cd v10 = (u.eps - (ky * ky) * imu) * ik;
+  const cd v11 = (ky * kx) * imu * ik;
   V[k + (long long)k * n] = v00;
   V[k + (long long)(h + k) * n] = v01;
   V[(h + k) + (long long)k * n] = v10;
@@ -646,7 +699,7 @@ __global__ __launch_bounds__(256) void k_rt(const ConvGeom g, const RtParam* par
   {
     double kx, ky;
     k_vectors(g, pp, i0, kx, ky);
-    const cd kzi = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
+    const cd kzi = -csqrt_principal(p.eps_ref * p.mu_ref - mk(kx * kx + ky * ky, 0.0));
     kzinc = fabs(kzi.re);
   }
   for (int k = threadIdx.x; k < h; k += blockDim.x) {
@@ -661,12 +714,14 @@ __global__ __launch_bounds__(256) void k_rt(const ConvGeom g, const RtParam* par
     amp[7 * h + k] = T1[h + k];
     double kx, ky;
     k_vectors(g, pp, k, kx, ky);
-    const cd kzr = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
-    const cd kzt = csqrt_principal(p.eps_trn - mk(kx * kx + ky * ky, 0.0));
+    const cd kzr = -csqrt_principal(p.eps_ref * p.mu_ref - mk(kx * kx + ky * ky, 0.0));
+    const cd kzt = csqrt_principal(p.eps_trn * p.mu_trn - mk(kx * kx + ky * ky, 0.0));
     if ((kzr.re == 0.0 && kzr.im == 0.0) || (kzt.re == 0.0 && kzt.im == 0.0)) s_graze = 1;
   }
   __syncthreads();
   const bool graze = s_graze || (kzinc < 1e-12);
+  // |mu_ref / mu_trn| (rcwa/RCWASolver.cpp:1903-1907)
+  const double mu_ratio = sqrt(norm2(p.mu_ref) / norm2(p.mu_trn));
   if (threadIdx.x == 0 && graze) info[b] |= ORCWA_STATUS_GRAZING;
   for (int pol = 0; pol < max_pol; ++pol) {
     double* ro = ref_out + ((long long)b * max_pol + pol) * h;
@@ -685,14 +740,14 @@ __global__ __launch_bounds__(256) void k_rt(const ConvGeom g, const RtParam* par
       const cd ty = px * T0[h + k] + py * T1[h + k];
       double kx, ky;
       k_vectors(g, pp, k, kx, ky);
-      const cd kzr = -csqrt_principal(p.eps_ref - mk(kx * kx + ky * ky, 0.0));
-      const cd kzt = csqrt_principal(p.eps_trn - mk(kx * kx + ky * ky, 0.0));
+      const cd kzr = -csqrt_principal(p.eps_ref * p.mu_ref - mk(kx * kx + ky * ky, 0.0));
+      const cd kzt = csqrt_principal(p.eps_trn * p.mu_trn - mk(kx * kx + ky * ky, 0.0));
       const cd rz = cdiv(kx * rx + ky * ry, kzr);
       const cd tz = cdiv(kx * tx + ky * ty, kzt);
       const double rr = norm2(rx) + norm2(ry) + norm2(rz);
       const double tt = norm2(tx) + norm2(ty) + norm2(tz);
       ro[k] = fabs(kzr.re) * rr / kzinc;
-      to[k] = fabs(kzt.re) * tt / kzinc;
+      to[k] = mu_ratio * fabs(kzt.re) * tt / kzinc;
     }
   }
 }
@@ -743,6 +798,22 @@ void launch_build_v(hipStream_t s, const ConvGeom& g, const ProbParam* params, c
   const long long n = 2LL * g.nx * g.ny;
   hipLaunchKernelGGL(k_build_v, dim3(blocks_for(n * n), nprob), dim3(256), 0, s, g, params, W,
                      QW1, lam, keff, V, info, inst_of);
+}
+
+void launch_build_pq_mag(hipStream_t s, const ConvGeom& g, const ProbParam* params,
+                         const MagMats* mm, cd* const* P, cd* const* Q, int nprob) {
+  if (nprob <= 0) return;
+  const long long hh = (long long)g.nx * g.ny * g.nx * g.ny;
+  hipLaunchKernelGGL(k_build_pq_mag, dim3(blocks_for(hh), nprob), dim3(256), 0, s, g, params, mm,
+                     P, Q);
+}
+
+void launch_build_v_mag(hipStream_t s, int n, const cd* const* QW, const cd* lam, cd* keff,
+                        cd* const* V, int* info, const int* inst_of, int nprob) {
+  if (nprob <= 0) return;
+  const long long nn = (long long)n * n;
+  hipLaunchKernelGGL(k_build_v_mag, dim3(blocks_for(nn), nprob), dim3(256), 0, s, n, QW, lam,
+                     keff, V, info, inst_of);
 }
 
 void launch_zero_mats(hipStream_t s, long long count, cd* const* A, int batch) {

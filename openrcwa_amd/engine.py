@@ -73,7 +73,7 @@ class Plan:
     whole batch on the GPU."""
 
     def __init__(self, ctx: Context, nX: int, nY: int, coord_x, coord_y, thickness,
-                 lam, kx0, ky0, eps, pols):
+                 lam, kx0, ky0, eps, pols, mu=None):
         self._lib = _lib.load()
         self.ctx = ctx
         self.coord_x = np.ascontiguousarray(coord_x, dtype=np.float64)
@@ -92,6 +92,14 @@ class Plan:
         if eps.shape[1:] != (L, ncy - 1, ncx - 1) or eps.shape[0] not in (1, B):
             raise ValueError(f"eps must be (B|1, L, ncy-1, ncx-1); got {eps.shape}")
         self.eps = np.ascontiguousarray(eps)
+        self.mu = None
+        if mu is not None:
+            mu = np.asarray(mu, dtype=np.complex128)
+            if mu.ndim == 3:
+                mu = mu.reshape(1, L, ncy - 1, ncx - 1)
+            if mu.shape[1:] != (L, ncy - 1, ncx - 1) or mu.shape[0] not in (1, B):
+                raise ValueError(f"mu must be (B|1, L, ncy-1, ncx-1); got {mu.shape}")
+            self.mu = np.ascontiguousarray(mu)
         pols = np.asarray(pols, dtype=np.complex128)
         if pols.ndim == 2:
             pols = np.broadcast_to(pols, (B,) + pols.shape)
@@ -113,7 +121,8 @@ class Plan:
         inst = (_lib.OrcwaInstance * B)()
         eps_flat = self.eps.view(np.float64).reshape(self.eps.shape[0], -1)
         pol_flat = self.pols.view(np.float64).reshape(B, -1)
-        self._eps_flat, self._pol_flat = eps_flat, pol_flat
+        mu_flat = None if self.mu is None else self.mu.view(np.float64).reshape(self.mu.shape[0], -1)
+        self._eps_flat, self._pol_flat, self._mu_flat = eps_flat, pol_flat, mu_flat
         for i in range(B):
             inst[i].lam = float(lam[i])
             inst[i].kx0 = float(kx0[i])
@@ -121,6 +130,8 @@ class Plan:
             inst[i].eps = _dptr(eps_flat[0 if shared or self.eps.shape[0] == 1 else i])
             inst[i].n_pol = self.npol
             inst[i].pol = _dptr(pol_flat[i])
+            if mu_flat is not None:
+                inst[i].mu = _dptr(mu_flat[0 if mu_flat.shape[0] == 1 else i])
         self._inst = inst
         h = c_void_p()
         _lib.check(self._lib.orcwa_plan_create(ctx.handle, ctypes.byref(st), inst, B,
@@ -163,9 +174,9 @@ class Plan:
 
 
 def solve_batch(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps,
-                pols) -> BatchResult:
+                pols, mu=None) -> BatchResult:
     """Plan + execute + results (orcwa_gpu_solve_batch semantics)."""
-    p = Plan(ctx, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps, pols)
+    p = Plan(ctx, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps, pols, mu)
     try:
         p.execute()
         p.sync()

@@ -15,13 +15,13 @@
 * `main` is the `orcwa_rcwa` command (src/rcwa_Main.cpp:54-241): the same
   options for R/T and the same CSV.
 
+Magnetic materials (mu != 1, a magnetic conductivity) pass their
+permeability grids to the engine (rcwa/RCWADriver.cpp:129-149, 181-190).
 Not supported on this path (each is reported as an error, not approximated):
-magnetic materials (mu != 1 or a magnetic conductivity; the engine's P/Q
-kernels are the non-magnetic form of rcwa/Layer.cpp:75-253), the field and
-device images (`-field`, `-device`: rcwa/RCWASolver.cpp:1505-1775), and a
-patterned top or bottom slab (the engine's plane-wave kernel takes the
-half-spaces as homogeneous, which the reference driver assumes,
-rcwa/RCWADriver.h:11-13).
+the field and device images (`-field`, `-device`:
+rcwa/RCWASolver.cpp:1505-1775), and a patterned top or bottom slab (the
+engine's plane-wave kernel takes the half-spaces as homogeneous, which the
+reference driver assumes, rcwa/RCWADriver.h:11-13).
 """
 from __future__ import annotations
 
@@ -448,6 +448,24 @@ def eps_at(prob: RCWAProblem, xc: float, yc: float, zc: float, lam: float) -> co
     return e
 
 
+def mu_at(prob: RCWAProblem, xc: float, yc: float, zc: float, lam: float) -> complex:
+    """Relative permeability at a point (rcwa/RCWADriver.cpp:129-149): 1 in
+    the background, the last containing box's material otherwise, plus
+    +i msigma / (omega mu0) for a magnetic conductivity."""
+    m = 1.0 + 0.0j
+    for b in prob.boxes:
+        if not b.contains_xyz(xc, yc, zc):
+            continue
+        mi = b.material
+        if mi < 0 or mi >= len(prob.material_mu):
+            continue
+        m = complex(prob.material_mu[mi])
+        if mi < len(prob.material_mag_sigma) and prob.material_mag_sigma[mi] != 0.0:
+            omega = 2.0 * math.pi * C0_UM / lam
+            m += complex(0.0, prob.material_mag_sigma[mi] / (omega * MU0))
+    return m
+
+
 def has_magnetic_material(prob: RCWAProblem) -> bool:
     """rcwa/RCWADriver.cpp:152-160."""
     return (any(abs(complex(m) - 1.0) > 1e-12 for m in prob.material_mu)
@@ -485,9 +503,7 @@ def build_batch(prob: RCWAProblem) -> dict:
     zedges = derive_z_edges(prob)
     if len(zedges) < 2:
         raise OrcwaInputError("cannot form z slabs (check zmesh / geometry)")
-    if has_magnetic_material(prob):
-        raise OrcwaInputError("magnetic materials (mu != 1 or a magnetic conductivity) are "
-                              "not supported by the GPU engine")
+    magnetic = has_magnetic_material(prob)
     xg = common_x_grid(prob)
     yg = common_y_grid(prob)
     centres, thick = [], []
@@ -506,29 +522,38 @@ def build_batch(prob: RCWAProblem) -> dict:
     xc = [0.5 * (xg[i] + xg[i + 1]) for i in range(len(xg) - 1)]
     yc = [0.5 * (yg[j] + yg[j + 1]) for j in range(len(yg) - 1)]
     eps = np.empty((B, L, len(yc), len(xc)), dtype=complex)
+    mu = np.ones((B, L, len(yc), len(xc)), dtype=complex) if magnetic else None
     for b, lam in enumerate(lams):
         for s, zc in enumerate(centres):
             for j, y in enumerate(yc):
                 for i, x in enumerate(xc):
                     eps[b, s, j, i] = eps_at(prob, x, y, zc, lam)
+                    if magnetic:
+                        mu[b, s, j, i] = mu_at(prob, x, y, zc, lam)
     for s in (0, L - 1):
-        g = eps[:, s]
-        if not np.all(g == g[:, :1, :1]):
-            raise OrcwaInputError("the top and bottom slabs must be homogeneous "
-                                  f"(slab {s} is patterned)")
+        for grid in (eps, mu):
+            if grid is None:
+                continue
+            g = grid[:, s]
+            if not np.all(g == g[:, :1, :1]):
+                raise OrcwaInputError("the top and bottom slabs must be homogeneous "
+                                      f"(slab {s} is patterned)")
     th = prob.theta * math.pi / 180.0
     ph = prob.phi * math.pi / 180.0
     k0 = 2.0 * math.pi / lams
     xm, ym = 0.5 * (prob.xmin + prob.xmax), 0.5 * (prob.ymin + prob.ymax)
     eps_inc = np.array([eps_at(prob, xm, ym, centres[0], lam) for lam in lams])
-    n_inc = np.maximum(np.sqrt(eps_inc).real, 1e-6)
+    mu_inc = (np.array([mu_at(prob, xm, ym, centres[0], lam) for lam in lams]) if magnetic
+              else np.ones_like(eps_inc))
+    # n = sqrt(eps mu) of the incidence medium (rcwa/RCWADriver.cpp:289-296)
+    n_inc = np.maximum(np.sqrt(eps_inc * mu_inc).real, 1e-6)
     if np.any(np.abs(eps_inc.imag) > 1e-6 * np.abs(eps_inc.real)):
         _warn("the incidence medium is lossy (R/T assume a lossless incidence medium)")
     kx0 = k0 * n_inc * math.sin(th) * math.cos(ph)
     ky0 = k0 * n_inc * math.sin(th) * math.sin(ph)
     px, py = incident_polarisation(prob)
     return dict(xgrid=np.asarray(xg), ygrid=np.asarray(yg), thickness=np.asarray(thick),
-                centres=np.asarray(centres), lam=lams, kx0=kx0, ky0=ky0, eps=eps,
+                centres=np.asarray(centres), lam=lams, kx0=kx0, ky0=ky0, eps=eps, mu=mu,
                 pol=np.array([[px, py]], dtype=complex), nX=prob.nHx, nY=prob.nHy)
 
 
@@ -548,7 +573,8 @@ def run_rcwa(prob: RCWAProblem, ctx=None) -> list:
         thick = bt["thickness"].copy()
         thick[0] = thick[-1] = 0.0
         res = engine.solve_batch(ctx, bt["nX"], bt["nY"], bt["xgrid"], bt["ygrid"], thick,
-                                 bt["lam"], bt["kx0"], bt["ky0"], bt["eps"], bt["pol"])
+                                 bt["lam"], bt["kx0"], bt["ky0"], bt["eps"], bt["pol"],
+                                 bt["mu"])
     finally:
         if own:
             ctx.close()

@@ -406,3 +406,57 @@ def test_redheffer_star_identity_and_associativity():
     right = o.redheffer_star_general(a, o.redheffer_star_general(b, c))
     for x, y in zip(left, right):
         assert np.abs(x - y).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# magnetic layers (rcwa/Layer.cpp:129-145, 150-164, 206-213;
+# rcwa/RCWASolver.cpp:1833-1907): closed-form magnetic Fresnel, and the
+# electromagnetic duality of a patterned layer (swapping the eps and mu grids
+# swaps TE and TM), which only holds if mu enters P where eps enters Q
+# ---------------------------------------------------------------------------
+def magnetic_fresnel_R(theta, e2, m2, pol):
+    t = math.radians(theta)
+    k1z, s = math.cos(t), math.sin(t)
+    k2z = cmath.sqrt(e2 * m2 - s * s)
+    r = (m2 * k1z - k2z) / (m2 * k1z + k2z) if pol == "te" else \
+        (e2 * k1z - k2z) / (e2 * k1z + k2z)
+    return abs(r) ** 2
+
+
+@pytest.mark.parametrize("e2,m2", [(4.0, 4.0), (2.25, 3.0), (3.0, 2.25), (2.0 + 0.3j, 1.5 + 0.2j)])
+@pytest.mark.parametrize("theta", [0.0, 30.0, 45.0, 60.0])
+def test_magnetic_fresnel(e2, m2, theta):
+    xe, ye = [0.0, 1.0], [0.0, 1.0]
+    eps = [np.array([[1.0]]), np.array([[e2]])]
+    mu = [np.array([[1.0]]), np.array([[m2]])]
+    REF, TRN, _ = o.solve_stack(2, 0, 1.0, 1.0, xe, ye, eps, [0.0, 0.0], 0.6, theta, 0.0,
+                                mu_layers=mu)
+    assert abs(REF[0].sum() - magnetic_fresnel_R(theta, e2, m2, "te")) < 1e-12
+    assert abs(REF[1].sum() - magnetic_fresnel_R(theta, e2, m2, "tm")) < 1e-12
+    if e2 == m2 and theta == 0.0:
+        assert REF[0].sum() < 1e-14        # impedance matched
+    if isinstance(e2, float) and isinstance(m2, float):
+        assert abs(REF[0].sum() + TRN[0].sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("nY,phi", [(0, 0.0), (2, 30.0)])
+def test_magnetic_duality_patterned(nY, phi):
+    # air / grating / self-dual substrate (eps = mu = 2): swapping the
+    # grating's eps and mu grids maps TE onto TM order by order
+    xe, ye = [0.0, 0.3, 1.0], [0.0, 0.6, 1.0]
+    ea = np.array([[3.0 + 0.1j, 1.0], [2.0, 1.2]])
+    ma = np.array([[1.5, 1.0], [1.0, 2.5 + 0.05j]])
+    def run(e, m):
+        eps = [np.full((2, 2), 1.0 + 0j), e, np.full((2, 2), 2.0 + 0j)]
+        mu = [np.full((2, 2), 1.0 + 0j), m, np.full((2, 2), 2.0 + 0j)]
+        return o.solve_stack(3, nY, 1.0, 1.0, xe, ye, eps, [0.0, 0.4, 0.0], 0.8, 25.0, phi,
+                             mu_layers=mu)
+    R1, T1, _ = run(ea, ma)
+    R2, T2, _ = run(ma, ea)
+    for a, b in ((0, 1), (1, 0)):
+        assert np.abs(R1[a] - R2[b]).max() < 1e-12
+        assert np.abs(T1[a] - T2[b]).max() < 1e-12
+    # and a non-magnetic grating is not self-dual (the check can fail)
+    R3, _, _ = run(ea, np.ones((2, 2), complex))
+    R4, _, _ = run(np.ones((2, 2), complex), ea)
+    assert abs(R3[0].sum() - R4[1].sum()) < 1e-12 and abs(R3[0].sum() - R3[1].sum()) > 1e-3

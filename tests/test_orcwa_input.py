@@ -260,10 +260,20 @@ def test_build_batch(tmp_path):
     assert abs(px) ** 2 + abs(py) ** 2 == pytest.approx(0.5 * math.cos(math.radians(30)) ** 2 + 0.5)
 
 
+def test_build_batch_magnetic(tmp_path):
+    mag = FRESNEL.replace("material = 1 2.25 0 1 0", "material = 1 2.25 0 1.5 2.0e3")
+    p = oc.parse_rcwa_input(write(tmp_path, mag))
+    bt = oc.build_batch(p)
+    assert bt["mu"].shape == bt["eps"].shape
+    lam = bt["lam"][0]
+    w = 2 * math.pi * 2.99792458e14 / lam
+    # substrate (bottom slab): mu = 1.5 + i msigma / (omega mu0); air above: 1
+    assert np.allclose(bt["mu"][0, 1], 1.5 + 1j * 2.0e3 / (w * 1.25663706212e-6))
+    assert np.all(bt["mu"][:, 0] == 1.0)
+    assert oc.build_batch(oc.parse_rcwa_input(write(tmp_path, FRESNEL)))["mu"] is None
+
+
 def test_build_batch_rejects(tmp_path):
-    mag = FRESNEL.replace("material = 1 2.25 0 1 0", "material = 1 2.25 0 1.5 0")
-    with pytest.raises(oc.OrcwaInputError, match="magnetic"):
-        oc.build_batch(oc.parse_rcwa_input(write(tmp_path, mag)))
     patterned = FRESNEL.replace("-2.5e-07 2.5e-07 -2.5e-07 2.5e-07 -3e-07 0",
                                 "-1e-07 1e-07 -2.5e-07 2.5e-07 -3e-07 0")
     with pytest.raises(oc.OrcwaInputError, match="homogeneous"):

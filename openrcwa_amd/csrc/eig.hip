@@ -647,15 +647,31 @@ __device__ unsigned long long g_qr_prof[24];
 #define QPROF_CNT(slot)
 #endif
 
-constexpr int QW = 64;
+// A 48-wide window with 12 shifts keeps the LDS of one workgroup under 80 KB,
+// so two matrices run on each CU and one's latency-bound chase and window
+// Schur overlap the other's MFMA updates (1.28x over a 64-wide window with 16
+// shifts at one workgroup per CU on n = 450, though each matrix takes more
+// windows and sweeps).
+#ifndef ORCWA_QR_W
+#define ORCWA_QR_W 48
+#endif
+#ifndef ORCWA_QR_NS
+#define ORCWA_QR_NS 12
+#endif
+constexpr int QW = ORCWA_QR_W;   // window (a multiple of 16)
 constexpr int QLD = QW + 1;   // padded LDS leading dimension (bank spread)
-constexpr int QNS = 16;
+constexpr int QT = QW / 16;   // 16-wide blocks of the window
+constexpr int QNS = ORCWA_QR_NS;
+static_assert(QW % 16 == 0 && QW <= 64 && QNS <= 64, "QR window");
 constexpr int QSCHED = QNS * QW;
-constexpr int AED_NW = 24;   // early-deflation window
+#ifndef ORCWA_AED_NW
+#define ORCWA_AED_NW 24
+#endif
+constexpr int AED_NW = ORCWA_AED_NW;   // early-deflation window
 
 struct QrShared {
-  cd Hw[QLD * QLD];     // window / small block (column-major, ld QLD)
-  cd U[QLD * QLD];      // accumulated unitary; trailing block for the shifts
+  cd Hw[QLD * QW];      // window / small block (column-major, ld QLD)
+  cd U[QLD * QW];       // accumulated unitary; trailing block for the shifts
   short sched_b[QSCHED];
   short sched_p[QSCHED];
   short pass_start[QSCHED + 1];
@@ -666,6 +682,7 @@ struct QrShared {
   int bdone[QNS];
   cd c, s, sh;
   cd hv[QW];           // Householder vector (early deflation)
+  cd spike[QW];        // early-deflation spike, reduced to its first entry
   int nsched, npass, l, ok, ired[4];
 };
 
@@ -819,9 +836,9 @@ __device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long 
   constexpr int HALF = QW / 8;
   for (int j0 = c0 + 16 * wave; j0 < c1; j0 += 64) {
     const int j = j0 + (lane & 15);
-    v4d accP[4], accQ[4], accR[4];
+    v4d accP[QT], accQ[QT], accR[QT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < QT; ++t) {
       accP[t] = (v4d){0, 0, 0, 0};
       accQ[t] = (v4d){0, 0, 0, 0};
       accR[t] = (v4d){0, 0, 0, 0};
@@ -839,7 +856,7 @@ __device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long 
         const int k = 4 * (HALF * h + q) + (lane >> 4);
         const double x = cur[q].re, y = cur[q].im, xy = x + y;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < QT; ++t) {
           const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // A[i][k] = conj(U[k][i])
           const double ua = u.re, ub = -u.im;
           accP[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, xy, accP[t], 0, 0, 0);
@@ -849,7 +866,7 @@ __device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long 
       }
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < QT; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * t + (lane >> 4) + 4 * r;
@@ -868,9 +885,9 @@ __device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long l
   constexpr int HALF = QW / 8;
   for (int i0 = r0 + 16 * wave; i0 < r1; i0 += 64) {
     const int i = i0 + (lane & 15);
-    v4d accP[4], accQ[4], accR[4];
+    v4d accP[QT], accQ[QT], accR[QT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < QT; ++t) {
       accP[t] = (v4d){0, 0, 0, 0};
       accQ[t] = (v4d){0, 0, 0, 0};
       accR[t] = (v4d){0, 0, 0, 0};
@@ -888,7 +905,7 @@ __device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long l
         const int k = 4 * (HALF * h + q) + (lane >> 4);
         const double ya = cur[q].re, yb = cur[q].im, yab = ya + yb, yba = yb - ya;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < QT; ++t) {
           const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // B[k][j] = U[k][j]
           accP[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, u.re + u.im, accP[t], 0, 0, 0);
           accQ[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(yab, u.im, accQ[t], 0, 0, 0);
@@ -897,7 +914,7 @@ __device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long l
       }
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < QT; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int ii = i0 + (lane >> 4) + 4 * r;
@@ -910,7 +927,8 @@ __device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long l
 
 // load H[a:a+wa, a:a+wa] into S.Hw and reset S.U to the identity (zero padded)
 __device__ __forceinline__ void ms_load_window(QrShared& S, const cd* H, int n, int a, int wa) {
-  constexpr int PER = QW * QW / 256 / 2;
+  // QW*QW entries over 256 threads, loads issued in two batches
+  constexpr int PER = (QW * QW + 511) / 512;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     cd v[PER];
@@ -918,14 +936,17 @@ __device__ __forceinline__ void ms_load_window(QrShared& S, const cd* H, int n, 
     for (int t = 0; t < PER; ++t) {
       const int e = threadIdx.x + 256 * (PER * h + t);
       const int i = e % QW, j = e / QW;
-      v[t] = (i < wa && j < wa) ? H[(a + i) + (long long)(a + j) * n] : mk(0.0, 0.0);
+      v[t] = (e < QW * QW && i < wa && j < wa) ? H[(a + i) + (long long)(a + j) * n]
+                                              : mk(0.0, 0.0);
     }
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int e = threadIdx.x + 256 * (PER * h + t);
       const int i = e % QW, j = e / QW;
-      S.Hw[i + j * QLD] = v[t];
-      S.U[i + j * QLD] = (i == j && i < wa) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+      if (e < QW * QW) {
+        S.Hw[i + j * QLD] = v[t];
+        S.U[i + j * QLD] = (i == j && i < wa) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+      }
     }
   }
 }
@@ -1031,12 +1052,11 @@ __device__ int aed_window(QrShared& S, cd* H, cd* Z, int n, int hi, int nw, doub
     const cd sp = (ns == 0) ? mk(0.0, 0.0) : s;
     if (ns < nw || (sp.re == 0.0 && sp.im == 0.0)) {
       if (ns > 1 && !(sp.re == 0.0 && sp.im == 0.0)) {
-        // reflect the spike s conj(V(0, 0:ns)) onto its first entry (the
-        // reflector is built in column QLD-1 of the LDS array, outside every
-        // window), then return T(0:ns, 0:ns) to Hessenberg form
-        if (lane < ns) T[lane + (QLD - 1) * QLD] = sp * conj(V[lane * QLD]);
+        // reflect the spike s conj(V(0, 0:ns)) onto its first entry, then
+        // return T(0:ns, 0:ns) to Hessenberg form
+        if (lane < ns) S.spike[lane] = sp * conj(V[lane * QLD]);
         wave_fence();
-        const cd tau = wave_house_col(T, 0, QLD - 1, ns, S.hv);
+        const cd tau = wave_house_col(S.spike, 0, 0, ns, S.hv);
         wave_house_apply(T, V, nw, 0, ns, 0, tau, S.hv);
         for (int k = 0; k + 2 < ns; ++k) {
           const cd t2 = wave_house_col(T, k + 1, k, ns - k - 1, S.hv);
@@ -1061,10 +1081,13 @@ __device__ int aed_window(QrShared& S, cd* H, cd* Z, int n, int hi, int nw, doub
   return nw - code;
 }
 
-__global__ __launch_bounds__(256) void k_qr_ms(int n, cd* const* Hp, cd* const* Zp, int* info,
+#ifndef ORCWA_QR_WG_PER_CU
+#define ORCWA_QR_WG_PER_CU 2
+#endif
+static_assert(sizeof(QrShared) * ORCWA_QR_WG_PER_CU <= 160 * 1024, "QR LDS per CU");
+__global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* const* Hp, cd* const* Zp, int* info,
                                                int maxsweep) {
   __shared__ QrShared S;
-  __shared__ int s_red[256];
   const int b = blockIdx.x;
   cd* __restrict__ H = Hp[b];
   cd* __restrict__ Z = Zp[b];
@@ -1076,13 +1099,10 @@ __global__ __launch_bounds__(256) void k_qr_ms(int n, cd* const* Hp, cd* const* 
     int best = 0;
     for (int k = 1 + tid; k <= hi; k += 256)
       if (small_subdiag(H, n, k, 0, hi, smlnum)) best = max(best, k);
-    s_red[tid] = best;
+    for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off, 64));
+    if ((tid & 63) == 0) S.ired[tid >> 6] = best;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (tid < st) s_red[tid] = max(s_red[tid], s_red[tid + st]);
-      __syncthreads();
-    }
-    const int l = s_red[0];
+    const int l = max(max(S.ired[0], S.ired[1]), max(S.ired[2], S.ired[3]));
     __syncthreads();
     if (l > 0 && tid == 0) H[l + (long long)(l - 1) * n] = mk(0.0, 0.0);
     __syncthreads();

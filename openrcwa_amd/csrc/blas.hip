@@ -318,6 +318,122 @@ __global__ __launch_bounds__(256) void k_getf2_panel(int n, int k0, int nb, cd* 
   }
 }
 
+// The same panel factorization held in registers, for panels of at most
+// GF_T rows: thread t owns row k0 + t of the nb <= NB panel columns.  Per
+// column: the pivot search (wave shuffles, one barrier), the row exchange and
+// the pivot-row broadcast through LDS (one barrier), then the scaling and the
+// rank-1 update of the thread's own row; the panel is read and written once.
+// The column steps are instantiated one by one (template recursion) so that
+// every register index is a constant.
+constexpr int GF_T = 512;
+
+struct GetrfRegState {
+  cd (*s_row)[NB];          // [2][NB]: the pivot row, the displaced row c
+  double (*s_wv)[GF_T / 64];
+  int (*s_wi)[GF_T / 64];
+  int n, k0, nb, r, t, lane, wave, b;
+  bool live;
+  int* ipiv;
+  int* info;
+  const int* info_map;
+};
+
+template <int J>
+__device__ __forceinline__ void getrf_reg_step(const GetrfRegState& S, cd (&a)[NB]) {
+  if constexpr (J < NB) {
+    if (J < S.nb) {
+      const int c = S.k0 + J, par = J & 1;
+      const int r = S.r;
+      // pivot: max cabs1 over rows c..n-1, the first index on ties
+      double v = (S.live && S.t >= J) ? cabs1(a[J]) : -1.0;
+      int vi = r;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(vi, off, 64);
+        if (ov > v || (ov == v && oi < vi)) {
+          v = ov;
+          vi = oi;
+        }
+      }
+      if (S.lane == 0) {
+        S.s_wv[par][S.wave] = v;
+        S.s_wi[par][S.wave] = vi;
+      }
+      __syncthreads();
+      double bv = S.s_wv[par][0];
+      int bi = S.s_wi[par][0];
+#pragma unroll
+      for (int w = 1; w < GF_T / 64; ++w) {
+        const double ov = S.s_wv[par][w];
+        const int oi = S.s_wi[par][w];
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      // rows c and bi trade places; s_row[0] becomes row c for every thread
+      const int put = (r == bi) ? 0 : ((r == c && bi != c) ? 1 : -1);
+      if (put >= 0) {
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) S.s_row[put][jj] = a[jj];
+      }
+      __syncthreads();
+      const int take = (bi == c) ? -1 : ((r == c) ? 0 : ((r == bi) ? 1 : -1));
+      if (take >= 0) {
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) a[jj] = S.s_row[take][jj];
+      }
+      const cd pv = S.s_row[0][J];
+      const bool zero = (pv.re == 0.0 && pv.im == 0.0);
+      if (S.t == 0) {
+        S.ipiv[(long long)S.b * S.n + c] = bi;
+        if (zero) atomicOr(&S.info[S.info_map ? S.info_map[S.b] : S.b], ORCWA_STATUS_SINGULAR);
+      }
+      if (S.live && r > c) {
+        a[J] = zero ? mk(0.0, 0.0) : a[J] * cinv(pv);
+#pragma unroll
+        for (int jj = J + 1; jj < NB; ++jj) a[jj] -= a[J] * S.s_row[0][jj];
+      }
+      // s_row is rewritten by the next column's exchange
+      __syncthreads();
+      getrf_reg_step<J + 1>(S, a);
+    }
+  }
+}
+
+__global__ __launch_bounds__(GF_T) void k_getf2_reg(int n, int k0, int nb, cd* const* Ap,
+                                                    int* ipiv, int* info, const int* info_map) {
+  __shared__ cd s_row[2][NB];
+  __shared__ double s_wv[2][GF_T / 64];
+  __shared__ int s_wi[2][GF_T / 64];
+  GetrfRegState S;
+  S.s_row = s_row;
+  S.s_wv = s_wv;
+  S.s_wi = s_wi;
+  S.n = n;
+  S.k0 = k0;
+  S.nb = nb;
+  S.b = blockIdx.x;
+  S.t = threadIdx.x;
+  S.lane = S.t & 63;
+  S.wave = S.t >> 6;
+  S.r = k0 + S.t;
+  S.live = S.r < n;
+  S.ipiv = ipiv;
+  S.info = info;
+  S.info_map = info_map;
+  cd* __restrict__ A = Ap[S.b];
+  cd a[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    a[j] = (S.live && j < nb) ? A[S.r + (long long)(k0 + j) * n] : mk(0.0, 0.0);
+  getrf_reg_step<0>(S, a);
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    if (S.live && j < nb) A[S.r + (long long)(k0 + j) * n] = a[j];
+}
+
 // Apply row swaps ipiv[k0..k0+nb) (in order) to columns [c0, c1) of an
 // n-row matrix with leading dimension n (one thread per column).
 __global__ void k_laswp(int n, int ncols_total, int k0, int nb, int c0, int c1, cd* const* Ap,
@@ -480,8 +596,12 @@ void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,
   const cd* const* Ac = (const cd* const*)A;
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = (n - k0 < NB) ? (n - k0) : NB;
-    hipLaunchKernelGGL(k_getf2_panel, dim3(batch), dim3(256), 0, s, n, k0, nb, A, ipiv, info,
-                       info_map);
+    if (n - k0 <= GF_T)
+      hipLaunchKernelGGL(k_getf2_reg, dim3(batch), dim3(GF_T), 0, s, n, k0, nb, A, ipiv, info,
+                         info_map);
+    else
+      hipLaunchKernelGGL(k_getf2_panel, dim3(batch), dim3(256), 0, s, n, k0, nb, A, ipiv, info,
+                         info_map);
     // swaps on the columns left and right of the panel
     if (k0 > 0)
       hipLaunchKernelGGL(k_laswp_panel, dim3((k0 + LASWP_NC - 1) / LASWP_NC, batch), dim3(256), 0,

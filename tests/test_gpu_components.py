@@ -44,10 +44,12 @@ def test_zgemm_ops(gpu_ctx, tA, cA, tB, cB):
 
 @pytest.mark.parametrize("n,nrhs,batch", [(1, 1, 2), (5, 3, 2), (31, 31, 2), (32, 32, 2),
                                           (33, 7, 2), (100, 100, 2), (450, 450, 2),
-                                          (97, 40, 3), (450, 900, 3)])
+                                          (97, 40, 3), (450, 900, 3), (513, 9, 2),
+                                          (600, 20, 3)])
 def test_zgesv(gpu_ctx, n, nrhs, batch):
     # even batches apply the row interchanges of the solve as one gather,
-    # odd ones swap by swap (both paths of zgetrs_batched)
+    # odd ones swap by swap (both paths of zgetrs_batched); panels of more
+    # than 512 rows factor in global memory, shorter ones in registers
     A, B = crand(batch, n, n), crand(batch, n, nrhs)
     X, info = g.zgesv(gpu_ctx, A, B)
     assert not info.any()

@@ -23,6 +23,8 @@ order:
   a11 plane wave ............ rcwa/RCWASolver.cpp:1778-1789, rcwa/Layer.cpp:453-480
   a12 per-order R/T ......... rcwa/RCWASolver.cpp:1792-1908
   a13 .ofd bridge glue ...... sol/rcwa_bridge.cpp:52-141, 179-366
+  fields  mode amplitudes ... rcwa/RCWASolver.cpp:1505-1645 (evaluateIntermediateField)
+          field/device images rcwa/RCWASolver.cpp:707-1011, 1322-1487
 
 Third-party arithmetic: the reference calls LAPACKE ``zgeev`` (unpinned
 version) and Eigen3 (unpinned) ``BDCSVD``, ``CompleteOrthogonalDecomposition``
@@ -270,7 +272,8 @@ class Layer:
         return (cy[0], cy[1], cy[-2], cy[-1], cx[0], cx[1], cx[-2], cx[-1])
 
     def eps_at(self, x, y):
-        """Layer::eps(x, y) cell lookup (rcwa/Layer.cpp:366-413)."""
+        """Layer::eps(x, y) cell lookup (rcwa/Layer.cpp:383-428): x wrapped
+        into the period, y outside the cell range gives 0."""
         return self._cell_value(self.eps_grid, x, y)
 
     def mu_at(self, x, y):
@@ -282,6 +285,8 @@ class Layer:
             x += self.Lx
         while x > cx[-1]:
             x -= self.Lx
+        if y < cy[0] or y > cy[-1]:
+            return 0.0 + 0.0j
         ix = next(i for i in range(cx.size - 1) if x <= cx[i + 1])
         iy = next(j for j in range(cy.size - 1) if y <= cy[j + 1])
         return grid[iy, ix]
@@ -411,6 +416,9 @@ class RCWASolver:
         Tdd = np.eye(nD, dtype=complex)
         Rud = np.zeros((nD, nD), dtype=complex)
         I = np.eye(nD)
+        # kept for the field reconstruction (the reference's taus_,
+        # interfacesE_/H_ and interfaceMap_)
+        self.taus, self.iface_E, self.iface_H, self.imap = [], E, H, imap
         for i in range(nL - 1):
             idx = nL - i - 1
             L = self.layers[stack[idx]]
@@ -425,6 +433,7 @@ class RCWASolver:
             G = H[k] - H[k] @ wR
             tau = sla.lu_solve(sla.lu_factor(F + G, check_finite=False), I,
                                check_finite=False)
+            self.taus.append(tau)
             Rud = -2.0 * G @ tau + I
             Tdd = 2.0 * wT @ tau
         return Tdd, Rud
@@ -463,6 +472,207 @@ class RCWASolver:
             return None, None
         mu_ratio = abs(mu_ref / mu_trn)
         return (np.abs(kzr.real) * r / kzinc, mu_ratio * np.abs(kzt.real) * t / kzinc)
+
+
+# --------------------------------------------------------------------------
+# field reconstruction: mode amplitudes inside the stack and field images
+# --------------------------------------------------------------------------
+EX, EY, EZ, HX, HY, HZ = range(6)            # rcwa/defns.h:27-35
+SLICE_XY, SLICE_XZ, SLICE_YZ = range(3)      # rcwa/defns.h:37-42
+MODULATION, REALPART, IMAGPART = range(3)    # rcwa/defns.h:44-49
+
+
+def intermediate_field(solver, c_inc, stack, thickness):
+    """RCWASolver::evaluateIntermediateField, inputCoeffs form
+    (rcwa/RCWASolver.cpp:1505-1645).  Returns (c_m, c_p) for the interior
+    layers j = 1..L-2 (list index j-1): c_p the downward mode amplitudes at
+    the layer's top, c_m the upward ones at its bottom.  Interface
+    S-matrices from t11 = (E+H)/2, t12 = (E-H)/2 (:1574-1586), the RTCM
+    reflection matrices rebuilt from taus_ (:1593-1598), the input entries
+    below 1e-14 dropped as the driver does (rcwa/RCWADriver.cpp:344-346)."""
+    nD = 2 * solver.nx * solver.ny
+    nL = len(stack)
+    if nL < 3:
+        raise ValueError("Intermediate field requires at least one finite-thickness "
+                         "interior layer")
+    I = np.eye(nD)
+    S11, S12, S21, S22, phases, REFs = [], [], [], [], [], []
+    Rud = np.zeros((nD, nD), dtype=complex)
+    REFs.append(Rud)
+    for_inv_s22 = None
+    for i in range(nL - 1):
+        index = nL - i - 1
+        L = solver.layers[stack[index]]
+        phi = (np.ones(nD, dtype=complex) if i == 0
+               else np.exp(1j * L.gamma * thickness[index]))
+        phases.append(phi)
+        k = solver.imap[index - 1]
+        E, H = solver.iface_E[k], solver.iface_H[k]
+        t11 = 0.5 * (E + H)
+        t12 = 0.5 * (E - H)
+        lu = sla.lu_factor(t11, check_finite=False)
+        inv_t22 = sla.lu_solve(lu, I, check_finite=False)
+        inv_t22_t12 = sla.lu_solve(lu, t12, check_finite=False)
+        S11.append(t11 - t12 @ inv_t22_t12)
+        S12.append(t12 @ inv_t22)
+        S21.append(-inv_t22_t12)
+        S22.append(inv_t22)
+        if i == 0:
+            for_inv_s22 = t11
+        wR = phi[:, None] * Rud * phi[None, :]
+        G = H - H @ wR
+        Rud = -2.0 * G @ solver.taus[i] + I
+        REFs.append(Rud)
+    c_in = np.asarray(c_inc, dtype=complex)
+    inp = np.where(np.abs(c_in) > 1e-14, c_in, 0.0)
+    R = solver.Rud @ inp
+    T = solver.Tdd @ inp
+    nIn = nL - 2
+    c_m, c_p, d = [None] * nIn, [None] * nIn, [None] * nIn
+    c_m[nIn - 1] = S12[0] @ (for_inv_s22 @ T)
+    temp = cod_solve(S11[nL - 2], R - S12[nL - 2] @ inp)
+    c_p[0] = S21[nL - 2] @ temp + S22[nL - 2] @ inp
+    d[nIn - 1] = phases[nL - 2] * c_p[0]
+    for i in range(nIn - 2, -1, -1):
+        c_m[nIn - i - 2] = REFs[i + 2] @ d[i + 1]
+        tmp = cod_solve(S11[i + 1], c_m[nIn - i - 2] - S12[i + 1] @ d[i + 1])
+        c_p[nIn - i - 1] = S21[i + 1] @ tmp + S22[i + 1] @ d[i + 1]
+        d[i] = phases[i + 1] * c_p[nIn - i - 1]
+    return c_m, c_p
+
+
+def find_interior_layer(stack, thickness, z, closed=True):
+    """Layer search of saveFieldImage (closed interval, rcwa/RCWASolver.cpp:
+    788-802) or saveDeviceImage (half-open, :1396-1408): z from the top of
+    the first interior layer, downwards.  Returns (j, z0, z1) or None."""
+    z0 = 0.0
+    for j in range(1, len(stack) - 1):
+        z1 = z0 + thickness[j]
+        if (z0 <= z <= z1) if closed else (z0 <= z < z1):
+            return j, z0, z1
+        z0 = z1
+    return None
+
+
+def layer_harmonics(solver, c_m, c_p, stack, j, z_p, z_m, comp):
+    """Harmonics of one field component in interior layer j at z_p below its
+    top (z_m = z_p - thickness <= 0), the SET_HARMONICS_2D lambda of
+    rcwa/RCWASolver.cpp:781-895: u = c_p exp(i gamma z_p),
+    d = c_m exp(-i gamma z_m); [Ex; Ey] = W (u + d), [Hx; Hy] = V (u - d);
+    Hz = (i/k0^2)(Kx Ey - Ky Ex); Ez = i [[eps]]^-1 (Kx Hy - Ky Hx) with the
+    ContinuousXY convolution matrix (rcwa/Layer.cpp:275-283; its
+    colPivHouseholderQr solve is an exact solve here).  Returns the h-vector,
+    index k = m*ny + n."""
+    L = solver.layers[stack[j]]
+    h = solver.nx * solver.ny
+    u = c_p[j - 1] * np.exp(1j * z_p * L.gamma)
+    d = c_m[j - 1] * np.exp(-1j * z_m * L.gamma)
+    if comp in (EX, EY):
+        v = L.W @ (u + d)
+        return v[:h] if comp == EX else v[h:]
+    if comp in (HX, HY):
+        v = L.V @ (u - d)
+        return v[:h] if comp == HX else v[h:]
+    Kx, Ky = solver.Kx, solver.Ky
+    k0 = 2 * PI / L.lam
+    if comp == HZ:
+        E = L.W @ (u + d)
+        return 1j / (k0 * k0) * (Kx * E[h:] - Ky * E[:h])
+    if comp == EZ:
+        Hv = L.V @ (u - d)
+        rhs = Kx * Hv[h:] - Ky * Hv[:h]
+        fe33 = fourier_2d(L.coord_x, L.coord_y, L.eps_grid, solver.nx, solver.ny,
+                          CONTINUOUS_XY)
+        return 1j * np.linalg.solve(fe33, rhs)
+    raise ValueError("Evaluation for this component is not implemented!")
+
+
+def _image_axes(solver, stack, thickness, slice_type, nres1):
+    """Sampling of saveFieldImage / saveDeviceImage (rcwa/RCWASolver.cpp:
+    730-779, 1330-1380): cell-centred samples, nres2 = int(nres1 L2 / L1) >= 1."""
+    L0 = solver.layers[0]
+    if slice_type == SLICE_XZ:
+        s1, l1 = L0.coord_x[0], L0.Lx
+        s2, l2 = 0.0, float(sum(thickness[1:len(stack) - 1]))
+    elif slice_type == SLICE_YZ:
+        s1, l1 = L0.coord_y[0], L0.Ly
+        s2, l2 = 0.0, float(sum(thickness[1:len(stack) - 1]))
+    elif slice_type == SLICE_XY:
+        s1, l1 = L0.coord_x[0], L0.Lx
+        s2, l2 = L0.coord_y[0], L0.Ly
+    else:
+        raise ValueError("unknown slice")
+    nres2 = max(1, int(nres1 * l2 / l1))
+    st1, st2 = l1 / nres1, l2 / nres2
+    return (s1 + st1 * (np.arange(nres1) + 0.5), s2 + st2 * (np.arange(nres2) + 0.5))
+
+
+def field_image(solver, c_inc, stack, thickness, slice_type, coord, comp, nres1=500):
+    """RCWASolver::saveFieldImage, inputCoeffs form (rcwa/RCWASolver.cpp:
+    707-1011): the complex field on the slice (rows = second axis), before
+    writeFieldCSV takes |f|, Re f or Im f.  f(r) = sum_mn f_mn
+    exp(i (kx0 + 2 pi m / Lx) x + i (ky0 + 2 pi n / Ly) y); XZ/YZ rows run
+    down the interior layers, XY is the plane z = coord."""
+    c_m, c_p = intermediate_field(solver, c_inc, stack, thickness)
+    a1, a2 = _image_axes(solver, stack, thickness, slice_type, nres1)
+    nx, ny = solver.nx, solver.ny
+    Lx, Ly = solver.layers[0].Lx, solver.layers[0].Ly
+    mx = np.arange(nx) - (nx - 1) // 2
+    my = np.arange(ny) - (ny - 1) // 2
+    kxm = 2 * PI * mx / Lx + solver.kx0
+    kyn = 2 * PI * my / Ly + solver.ky0
+
+    def harm2d(z):
+        hit = find_interior_layer(stack, thickness, z)
+        if hit is None:
+            raise ValueError("slice outside the interior layers")
+        j, z0, z1 = hit
+        return layer_harmonics(solver, c_m, c_p, stack, j, z - z0, z - z1,
+                               comp).reshape(nx, ny)
+
+    out = np.zeros((a2.size, a1.size), dtype=complex)
+    if slice_type in (SLICE_XZ, SLICE_YZ):
+        for i, z in enumerate(a2):
+            H2 = harm2d(z)
+            if slice_type == SLICE_XZ:
+                one = H2 @ np.exp(1j * kyn * coord)
+                out[i] = np.exp(1j * np.outer(a1, kxm)) @ one
+            else:
+                one = H2.T @ np.exp(1j * kxm * coord)
+                out[i] = np.exp(1j * np.outer(a1, kyn)) @ one
+    else:
+        H2 = harm2d(coord)
+        wx = np.exp(1j * np.outer(a1, kxm))          # [x, m]
+        wy = np.exp(1j * np.outer(a2, kyn))          # [y, n]
+        out = wy @ H2.T @ wx.T
+    return out
+
+
+def device_image(solver, stack, thickness, slice_type, coord, nres1=2000):
+    """RCWASolver::saveDeviceImage (rcwa/RCWASolver.cpp:1322-1487): the layer's
+    permittivity at the sample points (Layer::eps cell lookup), layer search
+    on the half-open interval.  Returns the complex grid (rows = second axis)."""
+    a1, a2 = _image_axes(solver, stack, thickness, slice_type, nres1)
+    out = np.zeros((a2.size, a1.size), dtype=complex)
+
+    def layer_at(z):
+        hit = find_interior_layer(stack, thickness, z, closed=False)
+        if hit is None:
+            raise ValueError("Device image requires at least one finite-thickness "
+                             "interior layer!")
+        return solver.layers[stack[hit[0]]]
+
+    if slice_type in (SLICE_XZ, SLICE_YZ):
+        for i, z in enumerate(a2):
+            L = layer_at(z)
+            for j, v in enumerate(a1):
+                out[i, j] = L.eps_at(v, coord) if slice_type == SLICE_XZ else L.eps_at(coord, v)
+    else:
+        L = layer_at(coord)
+        for i, y in enumerate(a2):
+            for j, x in enumerate(a1):
+                out[i, j] = L.eps_at(x, y)
+    return out
 
 
 # --------------------------------------------------------------------------

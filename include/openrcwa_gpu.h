@@ -153,6 +153,39 @@ int orcwa_gpu_slab_smatrix(orcwa_gpu_context *ctx, int n, int batch, const doubl
 int orcwa_gpu_redheffer_star(orcwa_gpu_context *ctx, int n, int batch, const double *top,
                              const double *bottom, double *out, int *info);
 
+/* Field images inside the stack: the inputCoeffs form of
+ * RCWASolver::saveFieldImage (rcwa/RCWASolver.cpp:707-1011, mode amplitudes of
+ * evaluateIntermediateField :1505-1645) that the .orcwa driver calls for its
+ * first wavelength (rcwa/RCWADriver.cpp:342-352).  Components and slices keep
+ * the reference's numbering (rcwa/defns.h:27-42). */
+enum orcwa_field_component {
+  ORCWA_FIELD_EX = 0, ORCWA_FIELD_EY = 1, ORCWA_FIELD_EZ = 2,
+  ORCWA_FIELD_HX = 3, ORCWA_FIELD_HY = 4, ORCWA_FIELD_HZ = 5
+};
+enum orcwa_slice { ORCWA_SLICE_XY = 0, ORCWA_SLICE_XZ = 1, ORCWA_SLICE_YZ = 2 };
+
+typedef struct {
+  int component;             /* orcwa_field_component */
+  int slice;                 /* orcwa_slice */
+  double coord;              /* the fixed coordinate: y of an XZ slice, x of a YZ slice,
+                                z of an XY slice (measured down from the top of layer 1) */
+  int nres1;                 /* samples along the image columns; 0 = 500 as the reference */
+  int pol;                   /* which of the instance's polarisations excites the stack */
+} orcwa_field_request;
+
+/* Solve one instance and return its complex field on the slice: rows x cols
+ * values (re, im), row-major, rows along z (XZ, YZ) or y (XY), samples at cell
+ * centres start + step (i + 0.5) exactly as the reference.  Layer 0 and L-1
+ * are the half-spaces: XZ/YZ rows span the interior layers 1..L-2.  Ex, Ey,
+ * Hx, Hy are the modal fields ([Ex; Ey] = W (u + d), [Hx; Hy] = V (u - d));
+ * Ez = i [[eps]]^-1 (Kx Hy - Ky Hx) and Hz = (i / k0^2)(Kx Ey - Ky Ex) as the
+ * reference writes them (:849-891).  field == NULL returns only rows and cols.
+ * status (optional) gets the instance's orcwa_status bits.  At most 1024
+ * harmonics. */
+int orcwa_gpu_field_image(orcwa_gpu_context *ctx, const orcwa_stack *stack,
+                          const orcwa_instance *inst, const orcwa_field_request *req,
+                          int *rows, int *cols, double *field, int *status);
+
 /* Layout-compatible records of the .ofd RCWA input globals
  * (include/orcwa_rcwa.h:27-35 and :45-52 of the reference): same field order
  * and types, so rcwa_run reads the arrays the reference's input module fills. */

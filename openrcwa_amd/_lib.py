@@ -39,11 +39,21 @@ class OrcwaResult(ctypes.Structure):
                 ("amp", POINTER(c_double)), ("status", c_int)]
 
 
+class OrcwaFieldRequest(ctypes.Structure):
+    _fields_ = [("component", c_int), ("slice", c_int), ("coord", c_double),
+                ("nres1", c_int), ("pol", c_int)]
+
+
+FIELD_COMPONENTS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+SLICES = {"xy": 0, "xz": 1, "yz": 2}
+
+
 # every symbol include/openrcwa_gpu.h declares
 EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_solve_batch",
             "orcwa_plan_create", "orcwa_plan_execute", "orcwa_plan_sync",
             "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms",
-            "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "rcwa_run"]
+            "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "orcwa_gpu_field_image",
+            "rcwa_run"]
 
 _lib = None
 
@@ -73,6 +83,9 @@ def load(path: str | None = None):
     lib.orcwa_plan_destroy.argtypes = [c_void_p]
     lib.orcwa_plan_phase_ms.argtypes = [c_void_p, c_int]
     lib.orcwa_plan_phase_ms.restype = c_double
+    lib.orcwa_gpu_field_image.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
+                                          POINTER(OrcwaFieldRequest), POINTER(c_int),
+                                          POINTER(c_int), POINTER(c_double), POINTER(c_int)]
     _lib = lib
     return lib
 

@@ -22,8 +22,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unknown-warning-option"]
 
-SOURCES = ["blas.hip", "eig.hip", "rcwa_kernels.hip", "plan.hip", "smatrix.hip", "debug.hip",
-           "bridge.hip"]
+SOURCES = ["blas.hip", "eig.hip", "rcwa_kernels.hip", "field_kernels.hip", "plan.hip",
+           "smatrix.hip", "debug.hip", "bridge.hip"]
 
 
 def _run(cmd, verbose):

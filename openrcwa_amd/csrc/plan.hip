@@ -135,6 +135,15 @@ struct orcwa_plan {
   cd* d_amp = nullptr;             // [count][8][h]
   int* d_info = nullptr;           // [count]
 
+  // field plans (one instance): per-step interface matrices, the Tdd column
+  // chain at the top of every layer, and the instance's layer data
+  bool keep = false;
+  std::vector<cd*> kA, kB;         // [L-1] A = E + H, B = E - H of each cascade step
+  cd* kE_src = nullptr;            // the instance's E / H slots
+  cd* kH_src = nullptr;
+  cd* d_tv = nullptr;              // [L][2][n] chain columns i0, h+i0 at each layer top
+  std::vector<cd*> lW, lV, lK;     // per layer: W, V, keff (k0 units)
+
   hipEvent_t ev[8] = {};
   bool have_events = false;
   double phase_ms[6] = {-1, -1, -1, -1, -1, -1};
@@ -221,8 +230,8 @@ int orcwa_plan_destroy(orcwa_plan* p) {
   return ORCWA_OK;
 }
 
-int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
-                      int count, orcwa_plan** out) {
+static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
+                       int count, bool keep, orcwa_plan** out) {
   if (!out) return fail("orcwa_plan_create: null plan pointer");
   *out = nullptr;
   if (!ctx || !st || !inst) return fail("orcwa_plan_create: null argument");
@@ -239,9 +248,11 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   for (int i = 1; i < st->n_cy; ++i)
     if (!(st->coord_y[i] > st->coord_y[i - 1])) return fail("coord_y must be strictly increasing");
 
+  if (keep && count != 1) return fail("a field plan holds one instance");
   hipSetDevice(ctx->device);
   orcwa_plan* p = new orcwa_plan;
   p->ctx = ctx;
+  p->keep = keep;
   p->Nx = st->nx_harm;
   p->Ny = st->ny_harm;
   p->nx = nx;
@@ -446,6 +457,7 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   const size_t o_trn = take((size_t)count * p->maxpol * h * sizeof(double));
   const size_t o_amp = take((size_t)count * 8 * h * sizeof(cd));
   const size_t o_info = take((size_t)count * sizeof(int));
+  const size_t o_keep = take(keep ? 2 * (size_t)(L - 1) * slot + (size_t)L * 2 * n * sizeof(cd) : 0);
   const size_t ws_bytes = align256(off);
 
   if (hipMalloc(&p->d_ws, ws_bytes) != hipSuccess) {
@@ -478,6 +490,13 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   p->d_trn = (double*)(W + o_trn);
   p->d_amp = (cd*)(W + o_amp);
   p->d_info = (int*)(W + o_info);
+  if (keep) {
+    for (int si = 0; si < L - 1; ++si) {
+      p->kA.push_back((cd*)(W + o_keep + (size_t)(2 * si) * slot));
+      p->kB.push_back((cd*)(W + o_keep + (size_t)(2 * si + 1) * slot));
+    }
+    p->d_tv = (cd*)(W + o_keep + 2 * (size_t)(L - 1) * slot);
+  }
 
   // constant inputs
   hipStream_t s = ctx->stream;
@@ -654,6 +673,15 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
       int u = uidx[(size_t)ii * L + l];
       return u >= 0 ? p->d_binv + (size_t)u * 4 * h : nullptr;
     };
+    if (keep) {
+      p->kE_src = islot(0, 2);
+      p->kH_src = islot(0, 3);
+      for (int l = 0; l < L; ++l) {
+        p->lW.push_back(Wof(0, l));
+        p->lV.push_back(Vof(0, l));
+        p->lK.push_back(Kof(0, l));
+      }
+    }
     for (int sidx = 0; sidx < L - 1; ++sidx) {
       StepTables stt;
       const int idx = L - 1 - sidx, a = idx - 1, b = idx;
@@ -772,6 +800,11 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   HIPCHK(hipStreamSynchronize(s));
   *out = p;
   return ORCWA_OK;
+}
+
+int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
+                      int count, orcwa_plan** out) {
+  return plan_create(ctx, st, inst, count, false, out);
 }
 
 int orcwa_plan_execute(orcwa_plan* p) {
@@ -927,6 +960,10 @@ int orcwa_plan_execute(orcwa_plan* p) {
       }
       // A = E + H (in E), B = E - H (in H)
       launch_sum_diff(s, n, E, H, p->dev<const cd* const*>(st.keff_a), ni);
+      if (p->keep) {
+        HIPCHK(hipMemcpyAsync(p->kA[si], p->kE_src, nn * sizeof(cd), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(p->kB[si], p->kH_src, nn * sizeof(cd), hipMemcpyDeviceToDevice, s));
+      }
       // R~ = phi R phi, kept transposed (the scaling is symmetric in i, j)
       cd* const* Rcur = p->dev<cd* const*>(ct.Rt[si & 1]);
       cd* const* Rnext = p->dev<cd* const*>(ct.Rt[(si + 1) & 1]);
@@ -953,15 +990,24 @@ int orcwa_plan_execute(orcwa_plan* p) {
     {
       // columns i0, h+i0 of Rud and Tdd
       const int sl = p->L - 2;
+      // (a field plan keeps the chain's columns at the top of layer 1, 2, ..., L-1)
+      const size_t tvb = 2 * (size_t)n * sizeof(cd);
       launch_cascade_tail(s, n, i0, p->dev<const cd* const*>(ct.LU[sl]),
                           p->d_spiv + (size_t)sl * p->chunk * n, C(E), C(H),
                           sl > 0 ? p->dev<const cd* const*>(ct.Rt[sl & 1]) : nullptr, p->d_rtv,
                           ni);
-      for (int sc = sl - 1; sc >= 0; --sc)
+      if (p->keep)
+        HIPCHK(hipMemcpyAsync(p->d_tv + 2 * (size_t)n, p->d_rtv + 2 * (size_t)n, tvb,
+                              hipMemcpyDeviceToDevice, s));
+      for (int sc = sl - 1; sc >= 0; --sc) {
         launch_cascade_chain(s, n, p->dev<const cd* const*>(ct.LU[sc]),
                              p->d_spiv + (size_t)sc * p->chunk * n,
                              p->dev<const cd* const*>(ct.steps[sc + 1].keff_b),
                              p->dev<const double*>(ct.steps[sc + 1].k0d), p->d_rtv, ni);
+        if (p->keep)
+          HIPCHK(hipMemcpyAsync(p->d_tv + 2 * (size_t)n * (p->L - 1 - sc),
+                                p->d_rtv + 2 * (size_t)n, tvb, hipMemcpyDeviceToDevice, s));
+      }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(p->ev[4], s));
@@ -1049,6 +1095,168 @@ int orcwa_gpu_solve_batch(orcwa_gpu_context* ctx, const orcwa_stack* stack,
   if (rc == ORCWA_OK) rc = orcwa_plan_results(p, out);
   orcwa_plan_destroy(p);
   return rc;
+}
+
+// Field image of one instance: the inputCoeffs form of RCWASolver::saveFieldImage
+// that the .orcwa driver calls for its first wavelength (rcwa/RCWASolver.cpp:707-1011,
+// rcwa/RCWADriver.cpp:342-352).  The instance is solved by a field plan, then
+// the mode amplitudes of every interior layer follow from the kept cascade
+// data and the image is synthesised on the device.
+int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
+                          const orcwa_instance* inst, const orcwa_field_request* req, int* rows,
+                          int* cols, double* field, int* status) {
+  if (!ctx || !st || !inst || !req || !rows || !cols) return fail("field image: null argument");
+  const int L = st->n_layers;
+  if (L < 3 || !st->thickness)
+    return fail("field image: the stack needs at least one interior layer (3 layers)");
+  if (req->component < 0 || req->component > 5) return fail("field image: unknown component");
+  if (req->slice < 0 || req->slice > 2) return fail("field image: unknown slice");
+  if (req->nres1 < 0) return fail("field image: negative resolution");
+  if (req->pol < 0 || req->pol >= inst->n_pol) return fail("field image: no such polarisation");
+  if (st->n_cx < 2 || st->n_cy < 2 || !st->coord_x || !st->coord_y)
+    return fail("each axis needs at least two cell edges");
+  const int nx = 2 * st->nx_harm + 1, ny = 2 * st->ny_harm + 1, h = nx * ny, n = 2 * h;
+  if (n > 2048) return fail("field image: at most 1024 harmonics");
+  const int nres1 = req->nres1 > 0 ? req->nres1 : 500;       // rcwa/RCWASolver.cpp:763
+  const double Lx = st->coord_x[st->n_cx - 1] - st->coord_x[0];
+  const double Ly = st->coord_y[st->n_cy - 1] - st->coord_y[0];
+  double zint = 0.0;
+  for (int l = 1; l < L - 1; ++l) zint += st->thickness[l];
+  // image axes (rcwa/RCWASolver.cpp:730-779)
+  double s1, l1, s2, l2;
+  if (req->slice == 1) {
+    s1 = st->coord_x[0], l1 = Lx, s2 = 0.0, l2 = zint;
+  } else if (req->slice == 2) {
+    s1 = st->coord_y[0], l1 = Ly, s2 = 0.0, l2 = zint;
+  } else {
+    s1 = st->coord_x[0], l1 = Lx, s2 = st->coord_y[0], l2 = Ly;
+  }
+  const int nres2 = std::max(1, (int)(nres1 * l2 / l1));
+  const double step1 = l1 / nres1, step2 = l2 / nres2;
+  // layer of each harmonics row: closed interval [z0, z1] from the top of
+  // layer 1 (rcwa/RCWASolver.cpp:788-802)
+  const double k0 = 2.0 * kPi / inst->lambda;
+  auto locate = [&](double z, FieldRow& fr) {
+    double z0 = 0.0;
+    for (int j = 1; j < L - 1; ++j) {
+      const double z1 = z0 + st->thickness[j];
+      if (z0 <= z && z <= z1) {
+        fr.layer = j;
+        fr.pad = 0;
+        fr.k0zp = k0 * (z - z0);
+        fr.k0zm = k0 * (z - z1);
+        return true;
+      }
+      z0 = z1;
+    }
+    return false;
+  };
+  std::vector<FieldRow> frows;
+  if (req->slice == 0) {
+    FieldRow fr;
+    if (!locate(req->coord, fr)) return fail("field image: z of the XY slice is outside the interior layers");
+    frows.push_back(fr);
+  } else {
+    for (int r = 0; r < nres2; ++r) {
+      FieldRow fr;
+      if (!locate(s2 + step2 * (r + 0.5), fr)) return fail("field image: row outside the interior layers");
+      frows.push_back(fr);
+    }
+  }
+  *rows = nres2;
+  *cols = nres1;
+  if (!field) return ORCWA_OK;
+
+  orcwa_plan* p = nullptr;
+  int rc = plan_create(ctx, st, inst, 1, true, &p);
+  if (rc != ORCWA_OK) return rc;
+  struct Guard {
+    orcwa_plan* p;
+    void* buf = nullptr;
+    ~Guard() {
+      if (buf) hipFree(buf);
+      orcwa_plan_destroy(p);
+    }
+  } guard{p};
+  rc = orcwa_plan_execute(p);
+  if (rc != ORCWA_OK) return rc;
+  hipStream_t s = ctx->stream;
+  const int nrh = (int)frows.size();
+  std::vector<FieldLayer> fl(L);
+  const bool etype = (req->component == 0 || req->component == 1 || req->component == 5);
+  for (int l = 0; l < L; ++l) {
+    fl[l].M = etype ? p->lW[l] : p->lV[l];
+    fl[l].keff = p->lK[l];
+    fl[l].m33inv = p->gid[l] >= 0 ? p->gM[p->gid[l]] : nullptr;
+    fl[l].eps = p->ueps[l];
+  }
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = align256(off);
+    off = o + bytes;
+    return o;
+  };
+  const size_t o_cp = take((size_t)L * n * sizeof(cd));
+  const size_t o_cm = take((size_t)L * n * sizeof(cd));
+  const size_t o_rows = take(frows.size() * sizeof(FieldRow));
+  const size_t o_layers = take(fl.size() * sizeof(FieldLayer));
+  const size_t o_X = take((size_t)nrh * n * sizeof(cd));
+  const size_t o_Hm = take((size_t)nrh * h * sizeof(cd));
+  const size_t o_img = take((size_t)nres2 * nres1 * sizeof(cd));
+  if (hipMalloc(&guard.buf, align256(off)) != hipSuccess) return ORCWA_ERR_NOMEM;
+  char* B = (char*)guard.buf;
+  cd* cp = (cd*)(B + o_cp);
+  cd* cm = (cd*)(B + o_cm);
+  FieldRow* d_rows = (FieldRow*)(B + o_rows);
+  FieldLayer* d_layers = (FieldLayer*)(B + o_layers);
+  cd* X = (cd*)(B + o_X);
+  cd* Hm = (cd*)(B + o_Hm);
+  cd* img = (cd*)(B + o_img);
+  HIPCHK(hipMemcpyAsync(d_rows, frows.data(), frows.size() * sizeof(FieldRow),
+                        hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_layers, fl.data(), fl.size() * sizeof(FieldLayer),
+                        hipMemcpyHostToDevice, s));
+  // downward amplitudes for the requested polarisation, then the upward ones
+  // from the exit side (no upward wave in the exit half-space)
+  const cd px = p->pol[((size_t)0 * kMaxPol + req->pol) * 2 + 0];
+  const cd py = p->pol[((size_t)0 * kMaxPol + req->pol) * 2 + 1];
+  launch_field_combine(s, n, L, p->d_tv, px, py, cp);
+  HIPCHK(hipMemsetAsync(cm + (size_t)(L - 1) * n, 0, (size_t)n * sizeof(cd), s));
+  for (int j = L - 2; j >= 1; --j) {
+    const int si = L - 2 - j;    // the cascade step of the interface j | j+1
+    const double k0d = (j + 1 == L - 1) ? 0.0 : k0 * st->thickness[j + 1];
+    launch_field_up(s, n, p->kA[si], p->kB[si], cp + (size_t)(j + 1) * n, cm + (size_t)(j + 1) * n,
+                    p->lK[j + 1], k0d, cm + (size_t)j * n);
+  }
+  launch_field_modes(s, n, d_layers, d_rows, nrh, cp, cm, etype ? 1 : -1, X);
+  FieldGeom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.component = req->component;
+  g.slice = req->slice;
+  g.Lx = Lx;
+  g.Ly = Ly;
+  g.kx0 = inst->kx0;
+  g.ky0 = inst->ky0;
+  g.k0 = k0;
+  g.start1 = s1;
+  g.step1 = step1;
+  g.start2 = s2;
+  g.step2 = step2;
+  g.coord = req->coord;
+  launch_field_component(s, g, d_layers, d_rows, nrh, X, Hm);
+  launch_field_synth(s, g, Hm, nres2, nres1, img);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipMemcpy(field, img, (size_t)nres2 * nres1 * sizeof(cd), hipMemcpyDeviceToHost));
+  if (status) {
+    orcwa_result r;
+    std::memset(&r, 0, sizeof(r));
+    rc = orcwa_plan_results(p, &r);
+    if (rc != ORCWA_OK) return rc;
+    *status = r.status;
+  }
+  return ORCWA_OK;
 }
 
 }  // extern "C"

@@ -72,6 +72,32 @@ struct RtParam {
   cd pol[8][2];        // (px, py)
 };
 
+// field reconstruction: one layer of the instance whose fields are imaged
+struct FieldLayer {
+  const cd* M;         // W (E-type components) or V (H-type), n x n column-major
+  const cd* keff;      // [n] modal keff in k0 units
+  const cd* m33inv;    // [[eps]]^-1 (h x h) of a patterned layer, else null
+  cd eps;              // permittivity of a homogeneous layer
+};
+
+// one image row: its layer and depth below the layer top (z_p) and above the
+// layer bottom (z_m <= 0), both times k0
+struct FieldRow {
+  int layer;
+  int pad;
+  double k0zp, k0zm;
+};
+
+struct FieldGeom {
+  int nx, ny;
+  int component;       // 0..5 = Ex, Ey, Ez, Hx, Hy, Hz (rcwa/defns.h:27-35)
+  int slice;           // 0 XY, 1 XZ, 2 YZ (rcwa/defns.h:37-42)
+  double Lx, Ly, kx0, ky0, k0;
+  double start1, step1;  // column samples start1 + step1 (c + 0.5)
+  double start2, step2;  // row samples of an XY slice
+  double coord;          // the fixed coordinate of an XZ (y) or YZ (x) slice
+};
+
 // host launchers (all asynchronous on stream s)
 void launch_conv_strip(hipStream_t s, const ConvGeom& g, const StripTask* tasks, int ntasks,
                        const cd* grids, cd* strips, size_t lds_bytes);
@@ -110,5 +136,14 @@ void launch_cascade_tail(hipStream_t s, int n, int i0, const cd* const* LUt, con
                          int batch);
 void launch_cascade_chain(hipStream_t s, int n, const cd* const* LUt, const int* piv,
                           const cd* const* keff, const double* k0d, cd* rt, int batch);
+void launch_field_combine(hipStream_t s, int n, int L, const cd* tv, cd px, cd py, cd* cp);
+void launch_field_up(hipStream_t s, int n, const cd* A, const cd* B, const cd* cp1, const cd* cm1,
+                     const cd* keff1, double k0d1, cd* cm);
+void launch_field_modes(hipStream_t s, int n, const FieldLayer* layers, const FieldRow* rows,
+                        int nrows, const cd* cp, const cd* cm, int sign, cd* X);
+void launch_field_component(hipStream_t s, const FieldGeom& g, const FieldLayer* layers,
+                            const FieldRow* rows, int nrows, const cd* X, cd* Hm);
+void launch_field_synth(hipStream_t s, const FieldGeom& g, const cd* Hm, int nrows, int ncols,
+                        cd* img);
 
 }  // namespace orcwa

@@ -185,6 +185,56 @@ def solve_batch(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0
         p.close()
 
 
+def field_image(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps, pol,
+                component, slice_type, coord, nres1=0, mu=None):
+    """Complex field of one instance on a slice (orcwa_gpu_field_image; the
+    inputCoeffs form of RCWASolver::saveFieldImage, rcwa/RCWASolver.cpp:707-1011).
+    eps (and mu): (L, ncy-1, ncx-1); pol = (px, py); component in "Ex".."Hz" or
+    0..5; slice_type in "xy", "xz", "yz" or 0..2.  Returns (image (rows, cols)
+    complex, status)."""
+    comp = _lib.FIELD_COMPONENTS.get(component, component)
+    sl = _lib.SLICES.get(slice_type, slice_type)
+    lib = _lib.load()
+    coord_x = np.ascontiguousarray(coord_x, dtype=np.float64)
+    coord_y = np.ascontiguousarray(coord_y, dtype=np.float64)
+    thickness = np.ascontiguousarray(thickness, dtype=np.float64)
+    L = thickness.size
+    eps = np.ascontiguousarray(np.asarray(eps, dtype=np.complex128).reshape(
+        L, coord_y.size - 1, coord_x.size - 1))
+    pols = np.ascontiguousarray(np.asarray(pol, dtype=np.complex128).reshape(1, 2))
+    st = _lib.OrcwaStack()
+    st.nx_harm, st.ny_harm = int(nX), int(nY)
+    st.n_cx, st.n_cy = coord_x.size, coord_y.size
+    st.coord_x, st.coord_y = _dptr(coord_x), _dptr(coord_y)
+    st.n_layers = L
+    st.thickness = _dptr(thickness)
+    inst = _lib.OrcwaInstance()
+    inst.lam, inst.kx0, inst.ky0 = float(lam), float(kx0), float(ky0)
+    eps_flat = eps.view(np.float64).ravel()
+    pol_flat = pols.view(np.float64).ravel()
+    inst.eps = _dptr(eps_flat)
+    inst.n_pol = 1
+    inst.pol = _dptr(pol_flat)
+    mu_flat = None
+    if mu is not None:
+        mu_flat = np.ascontiguousarray(np.asarray(mu, dtype=np.complex128).reshape(
+            L, coord_y.size - 1, coord_x.size - 1)).view(np.float64).ravel()
+        inst.mu = _dptr(mu_flat)
+    req = _lib.OrcwaFieldRequest(int(comp), int(sl), float(coord), int(nres1), 0)
+    rows, cols, status = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _lib.check(lib.orcwa_gpu_field_image(ctx.handle, ctypes.byref(st), ctypes.byref(inst),
+                                         ctypes.byref(req), ctypes.byref(rows),
+                                         ctypes.byref(cols), None, None),
+               "orcwa_gpu_field_image")
+    img = np.zeros((rows.value, cols.value), dtype=np.complex128)
+    _lib.check(lib.orcwa_gpu_field_image(ctx.handle, ctypes.byref(st), ctypes.byref(inst),
+                                         ctypes.byref(req), ctypes.byref(rows),
+                                         ctypes.byref(cols), _dptr(img.view(np.float64)),
+                                         ctypes.byref(status)),
+               "orcwa_gpu_field_image")
+    return img, int(status.value)
+
+
 def _cplx(a, shape):
     a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.complex128), shape))
     return a

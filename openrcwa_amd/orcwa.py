@@ -12,16 +12,21 @@
   polarisation 1-5, Bloch vector from the incidence slab); `run_rcwa` solves
   it on the GPU in one call and returns the reference's `RCWAResult` list
   (rcwa/RCWADriver.h:16-28).
+* `FieldRequest` and the field / device images of the first wavelength
+  (rcwa/RCWADriver.h:30-41, rcwa/RCWADriver.cpp:211-216, 342-359): the field
+  on the GPU (`engine.field_image`, RCWASolver::saveFieldImage), the
+  permittivity map from the layer grids (RCWASolver::saveDeviceImage,
+  rcwa/RCWASolver.cpp:1322-1487), both written as writeFieldCSV does
+  (rcwa/RCWASolver.cpp:25-50).
 * `main` is the `orcwa_rcwa` command (src/rcwa_Main.cpp:54-241): the same
-  options for R/T and the same CSV.
+  options and the same CSV files.
 
 Magnetic materials (mu != 1, a magnetic conductivity) pass their
 permeability grids to the engine (rcwa/RCWADriver.cpp:129-149, 181-190).
-Not supported on this path (each is reported as an error, not approximated):
-the field and device images (`-field`, `-device`:
-rcwa/RCWASolver.cpp:1505-1775), and a patterned top or bottom slab (the
-engine's plane-wave kernel takes the half-spaces as homogeneous, which the
-reference driver assumes, rcwa/RCWADriver.h:11-13).
+Not supported on this path (reported as an error, not approximated): a
+patterned top or bottom slab (the engine's plane-wave kernel takes the
+half-spaces as homogeneous, which the reference driver assumes,
+rcwa/RCWADriver.h:11-13).
 """
 from __future__ import annotations
 
@@ -557,13 +562,128 @@ def build_batch(prob: RCWAProblem) -> dict:
                 pol=np.array([[px, py]], dtype=complex), nX=prob.nHx, nY=prob.nHy)
 
 
-def run_rcwa(prob: RCWAProblem, ctx=None) -> list:
+@dataclass
+class FieldRequest:
+    """RCWAFieldRequest (rcwa/RCWADriver.h:30-41): the component, slice,
+    coordinate [um] and save option of the field image, and the two output
+    paths (empty = not written)."""
+    component: str = "Ez"
+    slice: str = "xz"
+    coord: float = 0.0
+    opt: str = "mod"
+    path: str = ""
+    device_path: str = ""
+
+    def wants_anything(self) -> bool:
+        return bool(self.path) or bool(self.device_path)
+
+
+def write_field_csv(path: str, field: np.ndarray, opt: str):
+    """writeFieldCSV (rcwa/RCWASolver.cpp:25-50): |f|, Re f or Im f, one row
+    per line, ", " between values, 8 significant digits."""
+    if opt == "re":
+        vals = field.real
+    elif opt == "im":
+        vals = field.imag
+    else:
+        vals = np.abs(field)
+    with open(path, "w") as f:
+        for row in vals:
+            f.write(", ".join(format(float(v), ".8g") for v in row) + "\n")
+
+
+def _image_samples(start: float, length: float, count: int) -> np.ndarray:
+    step = length / count
+    return start + step * (np.arange(count) + 0.5)
+
+
+def device_image(bt: dict, slice_type: str, coord: float, b: int = 0) -> np.ndarray:
+    """RCWASolver::saveDeviceImage (rcwa/RCWASolver.cpp:1322-1487) for instance
+    b: the layer permittivity at 2000 cell-centred samples per row; rows run
+    down the interior slabs (XZ, YZ; a row belongs to the slab with
+    z0 <= z < z1) or along y (XY, the slab at depth coord).  Layer::eps
+    wraps x into the period and gives 0 for y outside it
+    (rcwa/Layer.cpp:383-428)."""
+    xg, yg, thick = bt["xgrid"], bt["ygrid"], bt["thickness"]
+    L = thick.size
+    eps = bt["eps"][b]
+    Lx, Ly = xg[-1] - xg[0], yg[-1] - yg[0]
+    zint = float(sum(thick[1:L - 1]))
+    nres1 = 2000
+    if slice_type == "xy":
+        s1, l1, s2, l2 = xg[0], Lx, yg[0], Ly
+    elif slice_type == "yz":
+        s1, l1, s2, l2 = yg[0], Ly, 0.0, zint
+    else:
+        s1, l1, s2, l2 = xg[0], Lx, 0.0, zint
+    nres2 = max(1, int(nres1 * l2 / l1))
+    a1 = _image_samples(s1, l1, nres1)
+    a2 = _image_samples(s2, l2, nres2)
+
+    def slab_at(z):
+        z0 = 0.0
+        for j in range(1, L - 1):
+            z1 = z0 + thick[j]
+            if z0 <= z < z1:
+                return j
+            z0 = z1
+        raise OrcwaInputError("Device image requires at least one finite-thickness "
+                              "interior layer!")
+
+    def wrap_x(x):
+        while x < xg[0]:
+            x += Lx
+        while x > xg[-1]:
+            x -= Lx
+        return x
+
+    def lookup(grid, xs, ys):
+        xs = np.array([wrap_x(float(x)) for x in np.atleast_1d(xs)])
+        ys = np.atleast_1d(np.asarray(ys, dtype=float))
+        ix = np.searchsorted(xg[1:], xs, side="left")
+        iy = np.clip(np.searchsorted(yg[1:], ys, side="left"), 0, len(yg) - 2)
+        out = grid[iy, ix]
+        return np.where((ys < yg[0]) | (ys > yg[-1]), 0.0, out)
+
+    out = np.zeros((nres2, nres1), dtype=complex)
+    if slice_type == "xy":
+        g = eps[slab_at(coord)]
+        for i, y in enumerate(a2):
+            out[i] = lookup(g, a1, np.full(nres1, y))
+    else:
+        for i, z in enumerate(a2):
+            g = eps[slab_at(z)]
+            out[i] = (lookup(g, a1, np.full(nres1, coord)) if slice_type == "xz"
+                      else lookup(g, np.full(nres1, coord), a1))
+    return out
+
+
+def field_image(bt: dict, ctx, component: str, slice_type: str, coord: float,
+                b: int = 0) -> np.ndarray:
+    """The field image of instance b (saveFieldImage with the driver's
+    plane-wave coefficients, rcwa/RCWADriver.cpp:342-352), on the GPU."""
+    from . import engine
+    thick = bt["thickness"].copy()
+    thick[0] = thick[-1] = 0.0
+    mu = None if bt["mu"] is None else bt["mu"][b]
+    img, _ = engine.field_image(ctx, bt["nX"], bt["nY"], bt["xgrid"], bt["ygrid"], thick,
+                                bt["lam"][b], bt["kx0"][b], bt["ky0"][b], bt["eps"][b],
+                                bt["pol"][0], component, slice_type, coord, 0, mu)
+    return img
+
+
+def run_rcwa(prob: RCWAProblem, ctx=None, field_req: FieldRequest | None = None) -> list:
     """Solve every wavelength of `prob` as one GPU batch (runRCWA,
-    rcwa/RCWADriver.cpp:195-378, without the field images).  A grazing
-    instance returns empty per-order arrays and R = T = 0, as the reference's
-    silent return from scatterPlaneWave leaves REF/TRN empty."""
+    rcwa/RCWADriver.cpp:195-378).  A grazing instance returns empty
+    per-order arrays and R = T = 0, as the reference's silent return from
+    scatterPlaneWave leaves REF/TRN empty.  With a field request the images
+    of the first wavelength are written; they need an interior slab
+    (rcwa/RCWADriver.cpp:209-216)."""
     from . import engine
     from ._lib import STATUS_GRAZING
+    if field_req is not None and field_req.wants_anything() and len(derive_z_edges(prob)) - 1 < 3:
+        raise OrcwaInputError("field images need an interior layer "
+                              "(split zmesh / geometry into 3 or more slabs)")
     bt = build_batch(prob)
     own = ctx is None
     if own:
@@ -575,6 +695,13 @@ def run_rcwa(prob: RCWAProblem, ctx=None) -> list:
         res = engine.solve_batch(ctx, bt["nX"], bt["nY"], bt["xgrid"], bt["ygrid"], thick,
                                  bt["lam"], bt["kx0"], bt["ky0"], bt["eps"], bt["pol"],
                                  bt["mu"])
+        if field_req is not None and field_req.path:
+            write_field_csv(field_req.path, field_image(bt, ctx, field_req.component,
+                                                        field_req.slice, field_req.coord),
+                            field_req.opt)
+        if field_req is not None and field_req.device_path:
+            write_field_csv(field_req.device_path,
+                            device_image(bt, field_req.slice, field_req.coord), "re")
     finally:
         if own:
             ctx.close()
@@ -613,27 +740,71 @@ def write_csv(path: str, results, verbose: bool, n_orders: int):
             f.write(row + "\n")
 
 
-_USAGE = ("Usage : python -m openrcwa_amd.orcwa [-o <out.csv>] [-v] <datafile.orcwa>\n"
+_USAGE = ("Usage : python -m openrcwa_amd.orcwa [-o <out.csv>] [-v] [field options] "
+          "<datafile.orcwa>\n"
           "  -o <out.csv>  : output CSV file (default: <input>_rcwa.csv)\n"
-          "  -v            : verbose - also write per-order efficiencies\n")
+          "  -v            : verbose - also write per-order efficiencies\n"
+          "  field image of the first wavelength:\n"
+          "  -field <c>    : Ex|Ey|Ez|Hx|Hy|Hz\n"
+          "  -device       : also write the permittivity map (real part)\n"
+          "  -slice <p>    : xy|xz|yz (default xz)\n"
+          "  -scoord <v>   : slice position [um] (default 0)\n"
+          "  -sopt <o>     : mod|re|im (default mod)\n"
+          "  -fieldout <f> : field CSV file (default: <input>_field.csv)\n"
+          "  -deviceout <f>: device CSV file (default: <input>_device.csv)\n")
 
 
 def main(argv=None) -> int:
     """The orcwa_rcwa command (src/rcwa_Main.cpp:54-241) for R/T output."""
     argv = sys.argv[1:] if argv is None else list(argv)
     infile, outfile, verbose = "", "", False
+    req = FieldRequest()
+    want_field = want_device = False
+    field_out = device_out = ""
     i = 0
     while i < len(argv):
         a = argv[i]
-        if a == "-o" and i + 1 < len(argv):
+        more = i + 1 < len(argv)
+        if a == "-o" and more:
             outfile = argv[i + 1]
             i += 1
         elif a in ("-v", "--verbose"):
             verbose = True
-        elif a in ("-field", "-slice", "-scoord", "-sopt", "-fieldout", "-deviceout", "-device"):
-            print(f"*** {a}: field and device images are not available on the GPU path",
-                  file=sys.stderr)
-            return 1
+        elif a == "-field" and more:
+            i += 1
+            if argv[i] not in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+                print(f"*** invalid -field argument: {argv[i]}", file=sys.stderr)
+                return 1
+            req.component = argv[i]
+            want_field = True
+        elif a == "-slice" and more:
+            i += 1
+            if argv[i] not in ("xy", "xz", "yz"):
+                print(f"*** invalid -slice argument: {argv[i]}", file=sys.stderr)
+                return 1
+            req.slice = argv[i]
+        elif a == "-scoord" and more:
+            i += 1
+            try:
+                req.coord = _stod(argv[i])
+            except ValueError:
+                print(f"*** invalid -scoord argument: {argv[i]}", file=sys.stderr)
+                return 1
+        elif a == "-sopt" and more:
+            i += 1
+            if argv[i] not in ("mod", "re", "im"):
+                print(f"*** invalid -sopt argument: {argv[i]}", file=sys.stderr)
+                return 1
+            req.opt = argv[i]
+        elif a == "-fieldout" and more:
+            i += 1
+            field_out = argv[i]
+        elif a == "-device":
+            want_device = True
+        elif a == "-deviceout" and more:
+            i += 1
+            device_out = argv[i]
+            want_device = True
         elif a in ("--help", "-h"):
             print(_USAGE, end="")
             return 0
@@ -646,6 +817,10 @@ def main(argv=None) -> int:
     stem = infile[:infile.rfind(".")] if "." in infile else infile
     if not outfile:
         outfile = stem + "_rcwa.csv"
+    if want_field:
+        req.path = field_out or stem + "_field.csv"
+    if want_device:
+        req.device_path = device_out or stem + "_device.csv"
     try:
         prob = parse_rcwa_input(infile)
     except OrcwaInputError as e:
@@ -660,7 +835,7 @@ def main(argv=None) -> int:
     print(f"Wavelens = {len(prob.lambdas)} ({prob.lambdas[-1]:g} - {prob.lambdas[0]:g} um)")
     print("=== solving ===")
     try:
-        results = run_rcwa(prob)
+        results = run_rcwa(prob, field_req=req if req.wants_anything() else None)
     except OrcwaInputError as e:
         print(f"*** solve error: {e}", file=sys.stderr)
         return 1
@@ -672,6 +847,10 @@ def main(argv=None) -> int:
     print("*** warning: the HDF5 spectrum file is not written on this path (the CSV is complete)",
           file=sys.stderr)
     print(f"=== output -> {outfile} ===")
+    if want_field:
+        print(f"=== field  -> {req.path} ===")
+    if want_device:
+        print(f"=== device -> {req.device_path} ===")
     print("=== normal end ===")
     return 0
 

@@ -301,7 +301,102 @@ def test_command_writes_the_csv(gpu_ctx, tmp_path, capsys):
         assert v.size == 5 + 2 * n_ord
         assert np.allclose(v[:5], [r.lam, r.R, r.T, r.A, r.R + r.T], rtol=1e-7, atol=1e-15)
         assert np.allclose(v[5:5 + n_ord], r.REF_orders, rtol=1e-7, atol=1e-15)
-    # the default output name, and the options the GPU path does not provide
+    # the default output names
     assert oc.main([str(src)]) == 0
     assert (tmp_path / "cmd_rcwa.csv").exists()
-    assert oc.main(["-field", "Ez", str(src)]) == 1
+    assert oc.main(["-field", "Ez", "-device", str(src)]) == 0
+    assert (tmp_path / "cmd_field.csv").exists() and (tmp_path / "cmd_device.csv").exists()
+
+
+FIELD_SLAB = """OpenRCWA 4 2
+title = driver field output
+xmesh = -2.5e-07 10 2.5e-07
+ymesh = -2.5e-07 10 2.5e-07
+zmesh = -1e-06 10 0.0 10 3e-07 10 7e-07
+material = 1 2.25 0 1 0
+geometry = 2 1 -2.5e-07 2.5e-07 -2.5e-07 2.5e-07 0 3e-07
+planewave = 30 0 1
+pbc = 1 1 0
+rcwaorder = 4 0
+wavelength = 0.6
+end
+"""
+
+
+def read_csv(path):
+    return np.array([[float(v) for v in line.split(",")]
+                     for line in path.read_text().splitlines()])
+
+
+def test_driver_field_output(gpu_ctx, tmp_path, capsys):
+    # Tests 25 and 23 of tests/test_rcwa_input.cpp:1339-1398, 1213-1252: an
+    # oblique TM glass slab gives a finite, non-zero Ez image, and the three
+    # save options agree cell by cell
+    src = tmp_path / "slab.orcwa"
+    src.write_text(FIELD_SLAB)
+    ez = tmp_path / "ez.csv"
+    assert oc.main(["-field", "Ez", "-fieldout", str(ez), "-o", str(tmp_path / "r.csv"),
+                    str(src)]) == 0
+    assert f"=== field  -> {ez} ===" in capsys.readouterr().out
+    v = read_csv(ez)
+    assert v.shape == (300, 500) and 1e-3 < np.abs(v).max() < 1e100
+    r = (tmp_path / "r.csv").read_text().splitlines()[1].split(",")
+    assert abs(float(r[1]) + float(r[2]) - 1.0) < 2e-3
+    imgs = {}
+    for opt in ("mod", "re", "im"):
+        f = tmp_path / f"ex_{opt}.csv"
+        assert oc.main(["-field", "Ex", "-sopt", opt, "-fieldout", str(f),
+                        "-o", str(tmp_path / "r.csv"), str(src)]) == 0
+        imgs[opt] = read_csv(f)
+    assert imgs["mod"].shape == imgs["re"].shape == imgs["im"].shape
+    assert np.abs(imgs["mod"] ** 2 - imgs["re"] ** 2 - imgs["im"] ** 2).max() < 1e-6
+
+
+def test_field_request_without_interior_layer(gpu_ctx, tmp_path, capsys):
+    # Test 32 (tests/test_rcwa_input.cpp:1715-1768): an interface only
+    src = tmp_path / "two.orcwa"
+    src.write_text(interface(0.0, 0.0, 1))
+    f = tmp_path / "nope.csv"
+    assert oc.main(["-field", "Ez", "-fieldout", str(f), str(src)]) == 1
+    assert not f.exists()
+    assert "interior layer" in capsys.readouterr().err
+    prob, res = solve(gpu_ctx, tmp_path, interface(0.0, 0.0, 1), "two2.orcwa")
+    assert abs(res[0].R - 0.04) < 2e-3
+
+
+def test_device_output(gpu_ctx, tmp_path):
+    # Test 36 (tests/test_rcwa_input.cpp:1937-1990)
+    src = tmp_path / "dev.orcwa"
+    src.write_text(PATTERNED["conical_grating_te"].replace("planewave = 30 30 2",
+                                                           "planewave = 0 0 1"))
+    d = tmp_path / "dev.csv"
+    assert oc.main(["-deviceout", str(d), "-o", str(tmp_path / "r.csv"), str(src)]) == 0
+    v = read_csv(d)
+    assert v.shape == (1200, 2000)
+    assert abs(v.min() - 1.0) < 1e-6 and abs(v.max() - 2.25) < 1e-6
+
+
+@pytest.mark.parametrize("comp,sl,coord", [("Hy", "xz", 0.1), ("Ez", "yz", -0.05),
+                                           ("Ex", "xy", 0.12)])
+def test_field_csv_against_oracle(gpu_ctx, tmp_path, comp, sl, coord):
+    # the first wavelength of a conical grating, written with 8 digits
+    src = tmp_path / "g.orcwa"
+    src.write_text(PATTERNED["conical_grating_te"])
+    got = {}
+    for opt in ("re", "im"):
+        f = tmp_path / f"{opt}.csv"
+        assert oc.main(["-field", comp, "-slice", sl, "-scoord", str(coord), "-sopt", opt,
+                        "-fieldout", str(f), "-o", str(tmp_path / "r.csv"), str(src)]) == 0
+        got[opt] = read_csv(f)
+    prob = oc.parse_rcwa_input(str(src))
+    bt = oc.build_batch(prob)
+    thick = list(bt["thickness"])
+    _, _, s = o.solve_stack(prob.nHx, prob.nHy, prob.Lx, prob.Ly, bt["xgrid"], bt["ygrid"],
+                            list(bt["eps"][0]), thick, bt["lam"][0], prob.theta, prob.phi)
+    c = s.generate_horizontal_plane_wave(bt["pol"][0][0], bt["pol"][0][1], 0)
+    ids = {"Ex": o.EX, "Ey": o.EY, "Ez": o.EZ, "Hx": o.HX, "Hy": o.HY, "Hz": o.HZ}
+    sls = {"xy": o.SLICE_XY, "xz": o.SLICE_XZ, "yz": o.SLICE_YZ}
+    ref = o.field_image(s, c, list(range(len(thick))), thick, sls[sl], coord, ids[comp])
+    assert got["re"].shape == ref.shape
+    scale = np.abs(ref).max()
+    assert np.abs(got["re"] + 1j * got["im"] - ref).max() < 1e-7 * scale

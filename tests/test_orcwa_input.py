@@ -310,3 +310,57 @@ end
     assert bt["eps"].shape == (1, 2, 1, 1) and np.all(bt["eps"] == 2.25)
     assert np.allclose(bt["thickness"], [1.0, 1.0]) and np.allclose(bt["centres"], [0.2, 0.2])
 
+
+
+DEVICE = HEAD.format(title="device image") + """zmesh = -7e-07 10 -3e-07 10 0.0 10 7e-07
+material = 1 2.25 0 1 0
+geometry = 2 1 -2.5e-07 0.0 -2.5e-07 2.5e-07 -3e-07 0
+planewave = 0 0 1
+pbc = 1 1 0
+rcwaorder = 4 0
+wavelength = 0.6
+end
+"""
+
+
+def test_device_image(tmp_path):
+    # Test 36 of tests/test_rcwa_input.cpp:1937-1990: an air / glass grating
+    bt = oc.build_batch(oc.parse_rcwa_input(write(tmp_path, DEVICE)))
+    img = oc.device_image(bt, "xz", 0.0)
+    # one interior slab: the grating, 0.3 um under the air half-space
+    assert img.shape == (int(2000 * 0.3 / 0.5), 2000)
+    assert img.real.min() == 1.0 and img.real.max() == 2.25
+    assert np.all(img[:, :1000].real == 2.25) and np.all(img[:, 1000:].real == 1.0)
+    # XY through the grating; YZ at x = -0.1 (inside the glass) and outside y
+    xy = oc.device_image(bt, "xy", 0.1)
+    assert xy.shape == (2000, 2000) and np.all(xy[:, :1000].real == 2.25)
+    assert np.all(xy[:, 1000:].real == 1.0)
+    yz = oc.device_image(bt, "yz", -0.1)
+    assert np.all(yz.real == 2.25)
+    out = oc.device_image(bt, "xz", 0.4)          # y beyond the period: 0
+    assert np.all(out == 0)
+    with pytest.raises(oc.OrcwaInputError):
+        oc.device_image(bt, "xy", 0.9)            # below the interior slabs
+
+
+def test_field_csv_writer(tmp_path):
+    f = np.array([[3 + 4j, -1e-5 + 0j], [0.123456789 - 2j, complex(-0.0, 1.5e300)]])
+    p = tmp_path / "f.csv"
+    oc.write_field_csv(str(p), f, "mod")
+    assert p.read_text() == "5, 1e-05\n2.0038068, 1.5e+300\n"
+    oc.write_field_csv(str(p), f, "re")
+    assert p.read_text() == "3, -1e-05\n0.12345679, -0\n"
+    oc.write_field_csv(str(p), f, "im")
+    assert p.read_text() == "4, 0\n-2, 1.5e+300\n"
+
+
+def test_field_options(tmp_path, capsys):
+    path = write(tmp_path, FRESNEL)
+    assert oc.main(["-field", "Bz", path]) == 1
+    assert "invalid -field" in capsys.readouterr().err
+    assert oc.main(["-field", "Ez", "-slice", "zz", path]) == 1
+    assert oc.main(["-sopt", "abs", path]) == 1
+    # FRESNEL has two slabs only: a field request is an error before any solve
+    with pytest.raises(oc.OrcwaInputError):
+        oc.run_rcwa(oc.parse_rcwa_input(path), ctx=object(),
+                    field_req=oc.FieldRequest(path=str(tmp_path / "x.csv")))

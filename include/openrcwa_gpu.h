@@ -153,6 +153,45 @@ int orcwa_gpu_slab_smatrix(orcwa_gpu_context *ctx, int n, int batch, const doubl
 int orcwa_gpu_redheffer_star(orcwa_gpu_context *ctx, int n, int batch, const double *top,
                              const double *bottom, double *out, int *info);
 
+/* A z-varying structure between z0 (bottom) and z1 (top), for the
+ * midpoint-staircase integrator RCWAIntegrator::compute(N)
+ * (core/RCWAIntegrator.cpp:20-86) with a sampler of cell edges and
+ * permittivity per z (VarLayerSampler, core/VarLayerSampler.cpp:52-68; the
+ * periodic K of core/LayerSampler.cpp:114-144).  The caller evaluates its
+ * sampler at the N + 2 points the integrator takes, in this order: z1 (the top
+ * basis), the slab midpoints z0 + (i + 1/2) dz for i = 0..N-1 (dz = (z1 - z0)
+ * / N), then z0 (the bottom basis). */
+typedef struct {
+  int nx_harm, ny_harm;      /* orders -Nx..Nx, -Ny..Ny */
+  int n_cx, n_cy;            /* edges per sample; every sample has the same periods */
+  int n_slices;              /* N >= 1 */
+  double lambda;
+  double kx0, ky0;           /* Bloch offsets of K; the reference's sampler uses 0, 0 */
+  double z0, z1;             /* z0 < z1 */
+  const double *coord_x;     /* [N + 2][n_cx] sorted edges of each sample */
+  const double *coord_y;     /* [N + 2][n_cy] */
+  const double *eps;         /* [N + 2][(n_cy-1)*(n_cx-1)] complex, cell (iy, ix) at
+                                iy*(n_cx-1)+ix as orcwa_instance.eps */
+} orcwa_zstack;
+
+/* The integrator's S-matrix (Rud, Tuu, Rdu, Tdd: the accessors of
+ * core/RCWAIntegrator.h:29-32), its top side in the mode basis at z1 and its
+ * bottom side in the basis at z0, and those bases (W0/V0/gam0 at z0, W1/V1/gam1
+ * at z1, gam = keff of WaveEquationSolver).  Matrices [n][n] complex
+ * column-major, n = 2 (2Nx+1)(2Ny+1); vectors [n] complex; NULL skips an output.
+ * The mode bases are this engine's (homogeneous samples take W = I), so the
+ * matrices match the reference's up to the choice of basis: W0 Tdd W1^-1,
+ * W1 Rud W1^-1, W1 Tuu W0^-1 and W0 Rdu W0^-1 are basis-free. */
+typedef struct {
+  double *Rud, *Tuu, *Rdu, *Tdd;
+  double *W0, *V0, *W1, *V1;
+  double *gam0, *gam1;
+  int status;                /* orcwa_status bits */
+} orcwa_zresult;
+
+int orcwa_gpu_rcwa_integrate(orcwa_gpu_context *ctx, const orcwa_zstack *zs,
+                             orcwa_zresult *res);
+
 /* Field images inside the stack: the inputCoeffs form of
  * RCWASolver::saveFieldImage (rcwa/RCWASolver.cpp:707-1011, mode amplitudes of
  * evaluateIntermediateField :1505-1645) that the .orcwa driver calls for its

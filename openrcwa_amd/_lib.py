@@ -44,6 +44,18 @@ class OrcwaFieldRequest(ctypes.Structure):
                 ("nres1", c_int), ("pol", c_int)]
 
 
+class OrcwaZStack(ctypes.Structure):
+    _fields_ = [("nx_harm", c_int), ("ny_harm", c_int), ("n_cx", c_int), ("n_cy", c_int),
+                ("n_slices", c_int), ("lam", c_double), ("kx0", c_double), ("ky0", c_double),
+                ("z0", c_double), ("z1", c_double), ("coord_x", POINTER(c_double)),
+                ("coord_y", POINTER(c_double)), ("eps", POINTER(c_double))]
+
+
+class OrcwaZResult(ctypes.Structure):
+    _fields_ = [(k, POINTER(c_double)) for k in ("Rud", "Tuu", "Rdu", "Tdd", "W0", "V0", "W1",
+                                                 "V1", "gam0", "gam1")] + [("status", c_int)]
+
+
 FIELD_COMPONENTS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 SLICES = {"xy": 0, "xz": 1, "yz": 2}
 
@@ -53,7 +65,7 @@ EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_solve_batch",
             "orcwa_plan_create", "orcwa_plan_execute", "orcwa_plan_sync",
             "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms",
             "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "orcwa_gpu_field_image",
-            "rcwa_run"]
+            "orcwa_gpu_rcwa_integrate", "rcwa_run"]
 
 _lib = None
 
@@ -83,6 +95,8 @@ def load(path: str | None = None):
     lib.orcwa_plan_destroy.argtypes = [c_void_p]
     lib.orcwa_plan_phase_ms.argtypes = [c_void_p, c_int]
     lib.orcwa_plan_phase_ms.restype = c_double
+    lib.orcwa_gpu_rcwa_integrate.argtypes = [c_void_p, POINTER(OrcwaZStack),
+                                             POINTER(OrcwaZResult)]
     lib.orcwa_gpu_field_image.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
                                           POINTER(OrcwaFieldRequest), POINTER(c_int),
                                           POINTER(c_int), POINTER(c_double), POINTER(c_int)]

@@ -180,6 +180,7 @@ int orcwa_dbg_convolution(orcwa_gpu_context* ctx, int nx_harm, int ny_harm, int 
   g.Lx = coord_x[n_cx - 1] - coord_x[0];
   g.Ly = coord_y[n_cy - 1] - coord_y[0];
   g.cx = dcx; g.cy = dcy;
+  g.sx = 0; g.sy = 0;
   const int mdim = nx > ny ? nx : ny;
   const size_t lds = (size_t)(2 * mdim - 1 + mdim * mdim + (n_cx > n_cy ? n_cx : n_cy)) * sizeof(cd);
   launch_conv_strip(s, g, dtasks, (int)tasks.size(), grid.data, strips.data, lds);

@@ -5,6 +5,8 @@
 // sub-blocks of a batch of matrices can be addressed without new tables.
 #pragma once
 
+#include <vector>
+
 #include "orcwa_common.h"
 #include "../../include/openrcwa_gpu.h"
 
@@ -53,5 +55,20 @@ struct EigWorkspace {
 void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, cd* const* W,
                    cd* lam, const EigWorkspace& ws, int* info, double* resid,
                    cd* const* scratch, int batch, hipEvent_t* phase_events);
+
+// Modes (W, V, keff in k0 units) of every layer of one instance, each layer
+// with its own cell edges layer_cx [L][n_cx], layer_cy [L][n_cy] (same
+// periods): a plan that stops after the layer modes.  The device pointers
+// stay valid until layer_modes_release.  status: orcwa_status bits of the
+// eigensolves.
+struct LayerModes {
+  orcwa_plan* plan = nullptr;
+  int n = 0;
+  int status = 0;
+  std::vector<const cd*> W, V, keff;
+};
+int layer_modes(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
+                const double* layer_cx, const double* layer_cy, LayerModes& out);
+void layer_modes_release(LayerModes& m);
 
 }  // namespace orcwa

@@ -138,6 +138,11 @@ struct orcwa_plan {
   // field plans (one instance): per-step interface matrices, the Tdd column
   // chain at the top of every layer, and the instance's layer data
   bool keep = false;
+  // modes-only plans stop after the layer modes (no cascade, no R/T); their
+  // layers may each have their own cell edges (one edge set per grid)
+  bool modes_only = false;
+  bool per_grid_edges = false;
+  std::vector<double> grid_cx, grid_cy;  // [G][ncx], [G][ncy] with per-grid edges
   std::vector<cd*> kA, kB;         // [L-1] A = E + H, B = E - H of each cascade step
   cd* kE_src = nullptr;            // the instance's E / H slots
   cd* kH_src = nullptr;
@@ -173,6 +178,8 @@ ConvGeom geom_of(const orcwa_plan* p) {
   g.Ly = p->Ly;
   g.cx = p->d_cx;
   g.cy = p->d_cy;
+  g.sx = p->per_grid_edges ? p->ncx : 0;
+  g.sy = p->per_grid_edges ? p->ncy : 0;
   return g;
 }
 
@@ -231,7 +238,8 @@ int orcwa_plan_destroy(orcwa_plan* p) {
 }
 
 static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
-                       int count, bool keep, orcwa_plan** out) {
+                       int count, bool keep, orcwa_plan** out, bool modes_only = false,
+                       const double* layer_cx = nullptr, const double* layer_cy = nullptr) {
   if (!out) return fail("orcwa_plan_create: null plan pointer");
   *out = nullptr;
   if (!ctx || !st || !inst) return fail("orcwa_plan_create: null argument");
@@ -243,16 +251,22 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   if (!st->thickness) return fail("null thickness array");
   const int nx = 2 * st->nx_harm + 1, ny = 2 * st->ny_harm + 1;
   if (nx > 63 || ny > 63) return fail("at most 31 harmonics per axis are supported");
-  for (int i = 1; i < st->n_cx; ++i)
-    if (!(st->coord_x[i] > st->coord_x[i - 1])) return fail("coord_x must be strictly increasing");
-  for (int i = 1; i < st->n_cy; ++i)
-    if (!(st->coord_y[i] > st->coord_y[i - 1])) return fail("coord_y must be strictly increasing");
+  if (!layer_cx) {     // (per-layer edges are checked layer by layer below)
+    for (int i = 1; i < st->n_cx; ++i)
+      if (!(st->coord_x[i] > st->coord_x[i - 1])) return fail("coord_x must be strictly increasing");
+    for (int i = 1; i < st->n_cy; ++i)
+      if (!(st->coord_y[i] > st->coord_y[i - 1])) return fail("coord_y must be strictly increasing");
+  }
 
   if (keep && count != 1) return fail("a field plan holds one instance");
+  if ((layer_cx || layer_cy || modes_only) && count != 1)
+    return fail("a modes-only plan holds one instance");
   hipSetDevice(ctx->device);
   orcwa_plan* p = new orcwa_plan;
   p->ctx = ctx;
   p->keep = keep;
+  p->modes_only = modes_only;
+  p->per_grid_edges = (layer_cx != nullptr);
   p->Nx = st->nx_harm;
   p->Ny = st->ny_harm;
   p->nx = nx;
@@ -289,15 +303,47 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->eps_trn.resize(count);
   p->mu_ref.resize(count);
   p->mu_trn.resize(count);
-  auto grid_of = [&](const double* v) {
+  // a grid is its values (and, with per-layer edges, the layer's edges)
+  auto grid_of = [&](const double* v, int l) {
     std::vector<double> key(v, v + 2 * ncell);
+    if (layer_cx) {
+      key.insert(key.end(), layer_cx + (size_t)l * p->ncx, layer_cx + (size_t)(l + 1) * p->ncx);
+      key.insert(key.end(), layer_cy + (size_t)l * p->ncy, layer_cy + (size_t)(l + 1) * p->ncy);
+    }
     auto it = grid_index.find(key);
     if (it != grid_index.end()) return it->second;
     const int g = (int)grid_index.size();
     grid_index.emplace(std::move(key), g);
     for (int c = 0; c < ncell; ++c) p->grid_vals.push_back(mk(v[2 * c], v[2 * c + 1]));
+    if (layer_cx) {
+      p->grid_cx.insert(p->grid_cx.end(), layer_cx + (size_t)l * p->ncx,
+                        layer_cx + (size_t)(l + 1) * p->ncx);
+      p->grid_cy.insert(p->grid_cy.end(), layer_cy + (size_t)l * p->ncy,
+                        layer_cy + (size_t)(l + 1) * p->ncy);
+    }
     return g;
   };
+  if (layer_cx) {
+    if (!layer_cy) {
+      delete p;
+      return fail("per-layer edges need both axes");
+    }
+    for (int l = 0; l < L; ++l) {
+      const double* x = layer_cx + (size_t)l * p->ncx;
+      const double* y = layer_cy + (size_t)l * p->ncy;
+      // (a moving edge may meet its neighbour: an empty cell adds nothing to
+      // the Fourier series)
+      for (int i = 1; i < p->ncx; ++i)
+        if (!(x[i] >= x[i - 1])) { delete p; return fail("layer edges must be sorted"); }
+      for (int i = 1; i < p->ncy; ++i)
+        if (!(y[i] >= y[i - 1])) { delete p; return fail("layer edges must be sorted"); }
+      if (std::fabs((x[p->ncx - 1] - x[0]) - p->Lx) > 1e-12 * p->Lx ||
+          std::fabs((y[p->ncy - 1] - y[0]) - p->Ly) > 1e-12 * p->Ly) {
+        delete p;
+        return fail("every layer must have the same period");
+      }
+    }
+  }
   auto uniform_of = [&](const double* v) {
     for (int c = 1; c < ncell; ++c)
       if (v[2 * c] != v[0] || v[2 * c + 1] != v[1]) return false;
@@ -340,11 +386,11 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
         p->ueps[il] = mk(e[0], e[1]);
         if (m) p->umu[il] = mk(m[0], m[1]);
       } else {
-        p->gid[il] = grid_of(e);
-        if (!mu_one) p->mgid[il] = grid_of(m);
+        p->gid[il] = grid_of(e, l);
+        if (!mu_one) p->mgid[il] = grid_of(m, l);
       }
     }
-    if (p->gid[(size_t)i * L] >= 0 || p->gid[(size_t)i * L + L - 1] >= 0) {
+    if (!modes_only && (p->gid[(size_t)i * L] >= 0 || p->gid[(size_t)i * L + L - 1] >= 0)) {
       delete p;
       fprintf(stderr,
               "[openrcwa] the incidence and exit half-spaces must be homogeneous (instance %d)\n",
@@ -407,8 +453,9 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
     off = o + bytes;
     return o;
   };
-  const size_t o_cx = take(p->ncx * sizeof(double));
-  const size_t o_cy = take(p->ncy * sizeof(double));
+  const size_t nes = p->per_grid_edges ? (size_t)std::max(1, p->G) : 1;
+  const size_t o_cx = take(nes * p->ncx * sizeof(double));
+  const size_t o_cy = take(nes * p->ncy * sizeof(double));
   const size_t o_grids = take((size_t)std::max(1, G) * ncell * sizeof(cd));
   // strips: type-0 records per grid (ncy-1 of nx^2+My+Mx), type-1 (ncx-1 of ny^2+Mx)
   const long long rec0 = (long long)nx * nx + (2 * ny - 1) + (2 * nx - 1);
@@ -457,7 +504,9 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   const size_t o_trn = take((size_t)count * p->maxpol * h * sizeof(double));
   const size_t o_amp = take((size_t)count * 8 * h * sizeof(cd));
   const size_t o_info = take((size_t)count * sizeof(int));
-  const size_t o_keep = take(keep ? 2 * (size_t)(L - 1) * slot + (size_t)L * 2 * n * sizeof(cd) : 0);
+  const bool keep_cascade = keep && !modes_only;
+  const size_t o_keep =
+      take(keep_cascade ? 2 * (size_t)(L - 1) * slot + (size_t)L * 2 * n * sizeof(cd) : 0);
   const size_t ws_bytes = align256(off);
 
   if (hipMalloc(&p->d_ws, ws_bytes) != hipSuccess) {
@@ -490,7 +539,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->d_trn = (double*)(W + o_trn);
   p->d_amp = (cd*)(W + o_amp);
   p->d_info = (int*)(W + o_info);
-  if (keep) {
+  if (keep_cascade) {
     for (int si = 0; si < L - 1; ++si) {
       p->kA.push_back((cd*)(W + o_keep + (size_t)(2 * si) * slot));
       p->kB.push_back((cd*)(W + o_keep + (size_t)(2 * si + 1) * slot));
@@ -500,8 +549,17 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
 
   // constant inputs
   hipStream_t s = ctx->stream;
-  HIPCHK(hipMemcpyAsync(p->d_cx, p->cx.data(), p->ncx * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(p->d_cy, p->cy.data(), p->ncy * sizeof(double), hipMemcpyHostToDevice, s));
+  if (p->per_grid_edges && p->G > 0) {
+    HIPCHK(hipMemcpyAsync(p->d_cx, p->grid_cx.data(), p->grid_cx.size() * sizeof(double),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(p->d_cy, p->grid_cy.data(), p->grid_cy.size() * sizeof(double),
+                          hipMemcpyHostToDevice, s));
+  } else {
+    HIPCHK(hipMemcpyAsync(p->d_cx, p->cx.data(), p->ncx * sizeof(double), hipMemcpyHostToDevice,
+                          s));
+    HIPCHK(hipMemcpyAsync(p->d_cy, p->cy.data(), p->ncy * sizeof(double), hipMemcpyHostToDevice,
+                          s));
+  }
   if (G > 0) {
     HIPCHK(hipMemcpyAsync(p->d_grids, p->grid_vals.data(), (size_t)G * ncell * sizeof(cd),
                           hipMemcpyHostToDevice, s));
@@ -909,6 +967,11 @@ int orcwa_plan_execute(orcwa_plan* p) {
                      p->dev<cd* const*>(ct.uV), p->d_binv, p->d_info, nu);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(p->ev[3], s));
+    if (p->modes_only) {
+      HIPCHK(hipEventRecord(p->ev[4], s));
+      HIPCHK(hipEventRecord(p->ev[7], s));
+      continue;
+    }
 
     // ---- 2d-e. interfaces and the RTCM cascade, exit side first
     cd* const* E = p->dev<cd* const*>(ct.E);
@@ -1260,3 +1323,35 @@ int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
 }
 
 }  // extern "C"
+
+namespace orcwa {
+
+int layer_modes(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
+                const double* layer_cx, const double* layer_cy, LayerModes& out) {
+  orcwa_plan* p = nullptr;
+  int rc = plan_create(ctx, st, inst, 1, true, &p, true, layer_cx, layer_cy);
+  if (rc != ORCWA_OK) return rc;
+  rc = orcwa_plan_execute(p);
+  if (rc == ORCWA_OK) rc = orcwa_plan_sync(p);
+  orcwa_result r;
+  std::memset(&r, 0, sizeof(r));
+  if (rc == ORCWA_OK) rc = orcwa_plan_results(p, &r);
+  if (rc != ORCWA_OK) {
+    orcwa_plan_destroy(p);
+    return rc;
+  }
+  out.plan = p;
+  out.n = p->n;
+  out.status = r.status;
+  out.W.assign(p->lW.begin(), p->lW.end());
+  out.V.assign(p->lV.begin(), p->lV.end());
+  out.keff.assign(p->lK.begin(), p->lK.end());
+  return ORCWA_OK;
+}
+
+void layer_modes_release(LayerModes& m) {
+  orcwa_plan_destroy(m.plan);
+  m.plan = nullptr;
+}
+
+}  // namespace orcwa

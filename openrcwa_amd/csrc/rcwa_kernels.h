@@ -11,8 +11,9 @@ struct ConvGeom {
   int ncx, ncy;        // number of edges
   int ncell;           // (ncx-1)*(ncy-1)
   double Lx, Ly;       // periods
-  const double* cx;    // device [ncx]
-  const double* cy;    // device [ncy]
+  const double* cx;    // device [ncx], or [grids][ncx] with sx = ncx
+  const double* cy;    // device [ncy], or [grids][ncy] with sy = ncy
+  int sx, sy;          // edge stride per grid: 0 when every grid shares the edges
 };
 
 // one strip of one grid for the Li-rule series

@@ -59,6 +59,8 @@ __global__ __launch_bounds__(256) void k_conv_strip(const ConvGeom g, const Stri
   cd* out = strips + t.out_off;
   const int mdim = (t.type == 0) ? nxh : nyh;
   const int M = 2 * mdim - 1;
+  const double* cx = g.cx + (long long)t.grid * g.sx;   // this grid's cell edges
+  const double* cy = g.cy + (long long)t.grid * g.sy;
   cd* sF = sm;                 // [M] coefficients of 1/eps
   cd* sT = sm + M;             // [mdim*mdim] Toeplitz -> inverse
   cd* sf = sT + mdim * mdim;   // [max(ncx,ncy)] 1/eps values
@@ -73,8 +75,8 @@ __global__ __launch_bounds__(256) void k_conv_strip(const ConvGeom g, const Stri
   __syncthreads();
   for (int i = tid; i < M; i += 256) {
     const int m = i - (M - 1) / 2;
-    sF[i] = (t.type == 0) ? fourier_coeff(m, g.cx, sf, ncx - 1, g.Lx, 1)
-                          : fourier_coeff(m, g.cy, sf, ncy - 1, g.Ly, 1);
+    sF[i] = (t.type == 0) ? fourier_coeff(m, cx, sf, ncx - 1, g.Lx, 1)
+                          : fourier_coeff(m, cy, sf, ncy - 1, g.Ly, 1);
   }
   __syncthreads();
   // T[r,c] = F[(r-c) + mdim-1], column-major in LDS
@@ -143,15 +145,15 @@ __global__ __launch_bounds__(256) void k_conv_strip(const ConvGeom g, const Stri
   cd one = mk(1.0, 0.0);
   if (t.type == 0) {
     const int My = 2 * nyh - 1, Mx = 2 * nxh - 1;
-    double yy[2] = {g.cy[t.index], g.cy[t.index + 1]};
+    double yy[2] = {cy[t.index], cy[t.index + 1]};
     for (int i = tid; i < My; i += 256)
       out[mdim * mdim + i] = fourier_coeff(i - (My - 1) / 2, yy, &one, 1, g.Ly, 0);
     for (int i = tid; i < Mx; i += 256)
       out[mdim * mdim + My + i] =
-          fourier_coeff(i - (Mx - 1) / 2, g.cx, eps + t.index * (ncx - 1), ncx - 1, g.Lx, 1);
+          fourier_coeff(i - (Mx - 1) / 2, cx, eps + t.index * (ncx - 1), ncx - 1, g.Lx, 1);
   } else {
     const int Mx = 2 * nxh - 1;
-    double xx[2] = {g.cx[t.index], g.cx[t.index + 1]};
+    double xx[2] = {cx[t.index], cx[t.index + 1]};
     for (int i = tid; i < Mx; i += 256)
       out[mdim * mdim + i] = fourier_coeff(i - (Mx - 1) / 2, xx, &one, 1, g.Lx, 0);
   }

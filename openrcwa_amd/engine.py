@@ -235,6 +235,46 @@ def field_image(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0
     return img, int(status.value)
 
 
+def integrator_points(z0, z1, N):
+    """The z of the samples the integrator takes, in its order: z1, the slab
+    midpoints z0 + (i + 1/2) dz, z0 (core/RCWAIntegrator.cpp:22-75)."""
+    dz = (z1 - z0) / float(N)
+    return [z1] + [z0 + (i + 0.5) * dz for i in range(N)] + [z0]
+
+
+def rcwa_integrate(ctx: Context, nX, nY, sample, lam, z0, z1, N, kx0=0.0, ky0=0.0):
+    """A z-varying structure by the midpoint staircase (orcwa_gpu_rcwa_integrate;
+    RCWAIntegrator::compute(N), core/RCWAIntegrator.cpp:20-86).  sample(z)
+    returns (coord_x, coord_y, eps) of the cross-section at z (the
+    VarLayerSampler of core/VarLayerSampler.cpp:52-68 with periodic K); every
+    sample has the same number of edges and the same periods.  Returns a dict
+    with Rud, Tuu, Rdu, Tdd, W0, V0, W1, V1 (n x n) and gam0, gam1 (n), and
+    the status bits."""
+    lib = _lib.load()
+    pts = integrator_points(z0, z1, N)
+    smp = [sample(z) for z in pts]
+    cx = np.ascontiguousarray([np.asarray(s_[0], dtype=np.float64) for s_ in smp])
+    cy = np.ascontiguousarray([np.asarray(s_[1], dtype=np.float64) for s_ in smp])
+    eps = np.ascontiguousarray([np.asarray(s_[2], dtype=np.complex128).reshape(
+        cy.shape[1] - 1, cx.shape[1] - 1) for s_ in smp])
+    zs = _lib.OrcwaZStack(int(nX), int(nY), cx.shape[1], cy.shape[1], int(N), float(lam),
+                          float(kx0), float(ky0), float(z0), float(z1), _dptr(cx), _dptr(cy),
+                          _dptr(eps.view(np.float64)))
+    n = 2 * (2 * int(nX) + 1) * (2 * int(nY) + 1)
+    mats = {k: np.zeros((n, n), dtype=np.complex128)
+            for k in ("Rud", "Tuu", "Rdu", "Tdd", "W0", "V0", "W1", "V1")}
+    vecs = {k: np.zeros(n, dtype=np.complex128) for k in ("gam0", "gam1")}
+    res = _lib.OrcwaZResult()
+    for k, a in list(mats.items()) + list(vecs.items()):
+        setattr(res, k, _dptr(a.view(np.float64)))
+    _lib.check(lib.orcwa_gpu_rcwa_integrate(ctx.handle, ctypes.byref(zs), ctypes.byref(res)),
+               "orcwa_gpu_rcwa_integrate")
+    out = {k: a.T.copy() for k, a in mats.items()}    # column-major on the device
+    out.update(vecs)
+    out["status"] = int(res.status)
+    return out
+
+
 def _cplx(a, shape):
     a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.complex128), shape))
     return a

@@ -23,6 +23,7 @@ order:
   a11 plane wave ............ rcwa/RCWASolver.cpp:1778-1789, rcwa/Layer.cpp:453-480
   a12 per-order R/T ......... rcwa/RCWASolver.cpp:1792-1908
   a13 .ofd bridge glue ...... sol/rcwa_bridge.cpp:52-141, 179-366
+  integ.  z-varying slabs ... core/RCWAIntegrator.cpp:20-86, core/VarLayerSampler.cpp:52-68
   fields  mode amplitudes ... rcwa/RCWASolver.cpp:1505-1645 (evaluateIntermediateField)
           field/device images rcwa/RCWASolver.cpp:707-1011, 1322-1487
 
@@ -714,6 +715,82 @@ def redheffer_star(state, Rud, Tuu):
     """The accumulated (rud, tuu, rdu, tdd) above a symmetric slab
     (Rdu = Rud, Tdd = Tuu): redheffer_star_general(state, slab)."""
     return redheffer_star_general(state, (Rud, Tuu, Rud, Tuu))
+
+
+# --------------------------------------------------------------------------
+# z-varying structures: the midpoint-staircase integrator
+# --------------------------------------------------------------------------
+def wave_equation(P, Q, stats=None):
+    """WaveEquationSolver::solve (rcwa/WaveEquationSolver.cpp:24-37): modes of
+    P Q with keff = GammaSqrtOp(eig) in the units of P and Q (no 1/k0)."""
+    d, W = eig_backend(P @ Q, stats)
+    keff = gamma_sqrt(d)
+    return W, Q @ W @ np.diag(1.0 / keff), keff
+
+
+def sampler_pq(layer, lam, nX, nY, kx0=0.0, ky0=0.0):
+    """VarLayerSampler::sample (core/VarLayerSampler.cpp:52-68) on the
+    periodic branch of LayerSampler::evaluateKMatrices
+    (core/LayerSampler.cpp:114-144): P and Q of the layer at this z with
+    K = diag(2 pi m / Lx + kx0), diag(2 pi n / Ly + ky0)."""
+    nx, ny = 2 * nX + 1, 2 * nY + 1
+    m = np.repeat(np.arange(nx) - nX, ny).astype(float)
+    n = np.tile(np.arange(ny) - nY, nx).astype(float)
+    Kx = np.diag((2 * PI / layer.Lx) * m + kx0).astype(complex)
+    Ky = np.diag((2 * PI / layer.Ly) * n + ky0).astype(complex)
+    return layer.wave_eqn_coeff(Kx, Ky, lam, nx, ny)
+
+
+def integrator_sample_points(z0, z1, N):
+    """The z of every sample RCWAIntegrator::compute takes, in the order of
+    core/RCWAIntegrator.cpp:22-75: z1 (the top basis), the N midpoints
+    z0 + (i + 1/2) dz, then z0 (the bottom basis)."""
+    dz = (z1 - z0) / float(N)
+    return [z1] + [z0 + (i + 0.5) * dz for i in range(N)] + [z0]
+
+
+def rcwa_integrator(sample, lam, nX, nY, z0, z1, N, kx0=0.0, ky0=0.0):
+    """RCWAIntegrator::compute(N) (core/RCWAIntegrator.cpp:20-86): the
+    structure between z0 and z1, `sample(z)` -> Layer, as N midpoint slabs
+    in the basis of the modes at z1, each put on top of the accumulated
+    S-matrix by the Redheffer star (:57-67), then the bottom side taken into
+    the basis of the modes at z0 (:74-85).  Returns dict(Rud, Tuu, Rdu, Tdd,
+    W0, V0, gam0, W1, V1, gam1) with gam the raw keff of WaveEquationSolver."""
+    P1, Q1 = sampler_pq(sample(z1), lam, nX, nY, kx0, ky0)
+    W1, V1, g1 = wave_equation(P1, Q1)
+    n = W1.shape[0]
+    I = np.eye(n)
+    Tdd, Tuu = I.astype(complex), I.astype(complex)
+    Rud, Rdu = np.zeros((n, n), complex), np.zeros((n, n), complex)
+    dz = (z1 - z0) / float(N)
+    for i in range(N):
+        z = z0 + (i + 0.5) * dz
+        P, Q = sampler_pq(sample(z), lam, nX, nY, kx0, ky0)
+        W, V, g = wave_equation(P, Q)
+        rud, tuu = slab_scatter_matrix(W, V, g, W1, V1, dz, lam)
+        # the accessors Rdu() / Tdd() return Rud_ / Tuu_ (rcwa/RCWAScatterMatrix.h:44-47)
+        Rud, Tuu, Rdu, Tdd = redheffer_star_general((rud, tuu, rud, tuu), (Rud, Tuu, Rdu, Tdd))
+    P0, Q0 = sampler_pq(sample(z0), lam, nX, nY, kx0, ky0)
+    W0, V0, g0 = wave_equation(P0, Q0)
+    F = np.linalg.solve(W1, W0)
+    G = np.linalg.solve(V1, V0)
+    A, B = 0.5 * (F + G), 0.5 * (F - G)
+    lu = sla.lu_factor(A - Rdu @ B)
+    Rdu = sla.lu_solve(lu, Rdu @ A - B)
+    Tdd = sla.lu_solve(lu, Tdd)
+    Rud = Tuu @ B @ Tdd + Rud
+    Tuu = Tuu @ (B @ Rdu + A)
+    return dict(Rud=Rud, Tuu=Tuu, Rdu=Rdu, Tdd=Tdd, W0=W0, V0=V0, gam0=g0, W1=W1, V1=V1,
+                gam1=g1)
+
+
+def integrator_harmonic_operators(r):
+    """The integrator's S-matrix in harmonic (E-field) space, independent of
+    the mode bases: T_down = W0 Tdd W1^-1, R_top = W1 Rud W1^-1,
+    T_up = W1 Tuu W0^-1, R_bottom = W0 Rdu W0^-1."""
+    W0, W1 = r["W0"], r["W1"]
+    return (W0 @ r["Tdd"] @ np.linalg.inv(W1), W1 @ r["Rud"] @ np.linalg.inv(W1),
+            W1 @ r["Tuu"] @ np.linalg.inv(W0), W0 @ r["Rdu"] @ np.linalg.inv(W0))
 
 
 # --------------------------------------------------------------------------

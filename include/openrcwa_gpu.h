@@ -192,6 +192,34 @@ typedef struct {
 int orcwa_gpu_rcwa_integrate(orcwa_gpu_context *ctx, const orcwa_zstack *zs,
                              orcwa_zresult *res);
 
+/* A sampler of the cross-section at z (LayerSampler::sample,
+ * core/LayerSampler.h:52-54, returning the geometry instead of P and Q):
+ * writes n_cx and n_cy sorted edges (same periods at every z) and the
+ * (n_cy-1)*(n_cx-1) complex permittivities; returns 0 on success. */
+typedef int (*orcwa_sampler_fn)(void *user, double z, double *coord_x, double *coord_y,
+                                double *eps);
+
+typedef struct {
+  int nx_harm, ny_harm;
+  int n_cx, n_cy;
+  double lambda;
+  double kx0, ky0;           /* Bloch offsets of K (0, 0 in the reference's sampler) */
+  double z0, z1;
+  orcwa_sampler_fn sample;
+  void *user;
+} orcwa_zsampler;
+
+/* The adaptive first-order perturbation integrator RedhefferIntegrator::
+ * compute(p, maxN, pe) (core/RedhefferIntegrator.cpp:173-399, its active
+ * triple-division form): segments whose first-order terms exceed pe are cut in
+ * three.  Every segment is evaluated in the modes and P, Q of its midpoint
+ * (homogeneous cross-sections included, as the reference does), so the sampler
+ * is called at z0, z1, the midpoint and at the cuts.  max_segments > 0 stops
+ * with ORCWA_ERR_UNSUPPORTED beyond that many accepted segments.  Results as
+ * orcwa_gpu_rcwa_integrate; n_segments (optional) gets the reference's N(). */
+int orcwa_gpu_redheffer_integrate(orcwa_gpu_context *ctx, const orcwa_zsampler *zs, double pe,
+                                  int max_segments, orcwa_zresult *res, int *n_segments);
+
 /* Field images inside the stack: the inputCoeffs form of
  * RCWASolver::saveFieldImage (rcwa/RCWASolver.cpp:707-1011, mode amplitudes of
  * evaluateIntermediateField :1505-1645) that the .orcwa driver calls for its

@@ -56,6 +56,16 @@ class OrcwaZResult(ctypes.Structure):
                                                  "V1", "gam0", "gam1")] + [("status", c_int)]
 
 
+SAMPLER_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_double, POINTER(c_double), POINTER(c_double),
+                              POINTER(c_double))
+
+
+class OrcwaZSampler(ctypes.Structure):
+    _fields_ = [("nx_harm", c_int), ("ny_harm", c_int), ("n_cx", c_int), ("n_cy", c_int),
+                ("lam", c_double), ("kx0", c_double), ("ky0", c_double), ("z0", c_double),
+                ("z1", c_double), ("sample", SAMPLER_FN), ("user", c_void_p)]
+
+
 FIELD_COMPONENTS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 SLICES = {"xy": 0, "xz": 1, "yz": 2}
 
@@ -65,7 +75,7 @@ EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_solve_batch",
             "orcwa_plan_create", "orcwa_plan_execute", "orcwa_plan_sync",
             "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms",
             "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "orcwa_gpu_field_image",
-            "orcwa_gpu_rcwa_integrate", "rcwa_run"]
+            "orcwa_gpu_rcwa_integrate", "orcwa_gpu_redheffer_integrate", "rcwa_run"]
 
 _lib = None
 
@@ -97,6 +107,8 @@ def load(path: str | None = None):
     lib.orcwa_plan_phase_ms.restype = c_double
     lib.orcwa_gpu_rcwa_integrate.argtypes = [c_void_p, POINTER(OrcwaZStack),
                                              POINTER(OrcwaZResult)]
+    lib.orcwa_gpu_redheffer_integrate.argtypes = [c_void_p, POINTER(OrcwaZSampler), c_double,
+                                                  c_int, POINTER(OrcwaZResult), POINTER(c_int)]
     lib.orcwa_gpu_field_image.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
                                           POINTER(OrcwaFieldRequest), POINTER(c_int),
                                           POINTER(c_int), POINTER(c_double), POINTER(c_int)]

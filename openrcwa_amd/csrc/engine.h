@@ -60,15 +60,17 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
 // with its own cell edges layer_cx [L][n_cx], layer_cy [L][n_cy] (same
 // periods): a plan that stops after the layer modes.  The device pointers
 // stay valid until layer_modes_release.  status: orcwa_status bits of the
-// eigensolves.
+// eigensolves.  With want_pq every layer (homogeneous ones too) is solved
+// from its P and Q, which are returned as well (k0 units: P/k0^2, Q/k0^2).
 struct LayerModes {
   orcwa_plan* plan = nullptr;
   int n = 0;
   int status = 0;
-  std::vector<const cd*> W, V, keff;
+  std::vector<const cd*> W, V, keff, P, Q;
 };
 int layer_modes(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
-                const double* layer_cx, const double* layer_cy, LayerModes& out);
+                const double* layer_cx, const double* layer_cy, LayerModes& out,
+                bool want_pq = false);
 void layer_modes_release(LayerModes& m);
 
 }  // namespace orcwa

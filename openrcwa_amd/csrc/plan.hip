@@ -142,6 +142,13 @@ struct orcwa_plan {
   // layers may each have their own cell edges (one edge set per grid)
   bool modes_only = false;
   bool per_grid_edges = false;
+  // P/Q plans (modes-only, one instance): every layer, homogeneous or not,
+  // is solved as a patterned problem with P and Q built apart (the mu = 1
+  // form of the magnetic builder), and P and Q are kept per layer
+  bool keep_pq = false;
+  std::vector<cd*> lP, lQ;         // per layer (P/Q plans)
+  std::vector<cd*> kP;             // kept copies of P, [L] slots
+  std::vector<cd*> kP_src;         // the P slot each copy is taken from
   std::vector<double> grid_cx, grid_cy;  // [G][ncx], [G][ncy] with per-grid edges
   std::vector<cd*> kA, kB;         // [L-1] A = E + H, B = E - H of each cascade step
   cd* kE_src = nullptr;            // the instance's E / H slots
@@ -239,7 +246,8 @@ int orcwa_plan_destroy(orcwa_plan* p) {
 
 static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
                        int count, bool keep, orcwa_plan** out, bool modes_only = false,
-                       const double* layer_cx = nullptr, const double* layer_cy = nullptr) {
+                       const double* layer_cx = nullptr, const double* layer_cy = nullptr,
+                       bool keep_pq = false) {
   if (!out) return fail("orcwa_plan_create: null plan pointer");
   *out = nullptr;
   if (!ctx || !st || !inst) return fail("orcwa_plan_create: null argument");
@@ -266,6 +274,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->ctx = ctx;
   p->keep = keep;
   p->modes_only = modes_only;
+  p->keep_pq = keep_pq && modes_only && keep;
   p->per_grid_edges = (layer_cx != nullptr);
   p->Nx = st->nx_harm;
   p->Ny = st->ny_harm;
@@ -382,7 +391,13 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
       }
       const bool mu_one = !m || (uniform_of(m) && m[0] == 1.0 && m[1] == 0.0);
       const size_t il = (size_t)i * L + l;
-      if (uniform_of(e) && (!m || uniform_of(m))) {
+      if (p->keep_pq) {
+        // P and Q apart: the magnetic builder with a mu = 1 grid
+        std::vector<double> ones(2 * (size_t)ncell, 0.0);
+        for (int c = 0; c < ncell; ++c) ones[2 * c] = 1.0;
+        p->gid[il] = grid_of(e, l);
+        p->mgid[il] = grid_of(m ? m : ones.data(), l);
+      } else if (uniform_of(e) && (!m || uniform_of(m))) {
         p->ueps[il] = mk(e[0], e[1]);
         if (m) p->umu[il] = mk(m[0], m[1]);
       } else {
@@ -507,6 +522,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   const bool keep_cascade = keep && !modes_only;
   const size_t o_keep =
       take(keep_cascade ? 2 * (size_t)(L - 1) * slot + (size_t)L * 2 * n * sizeof(cd) : 0);
+  const size_t o_kp = take(p->keep_pq ? (size_t)L * slot : 0);
   const size_t ws_bytes = align256(off);
 
   if (hipMalloc(&p->d_ws, ws_bytes) != hipSuccess) {
@@ -539,6 +555,8 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->d_trn = (double*)(W + o_trn);
   p->d_amp = (cd*)(W + o_amp);
   p->d_info = (int*)(W + o_info);
+  if (p->keep_pq)
+    for (int l = 0; l < L; ++l) p->kP.push_back((cd*)(W + o_kp + (size_t)l * slot));
   if (keep_cascade) {
     for (int si = 0; si < L - 1; ++si) {
       p->kA.push_back((cd*)(W + o_keep + (size_t)(2 * si) * slot));
@@ -738,6 +756,14 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
         p->lW.push_back(Wof(0, l));
         p->lV.push_back(Vof(0, l));
         p->lK.push_back(Kof(0, l));
+        if (p->keep_pq) {
+          // P is built into the V slot (and later replaced by V); Q stays in
+          // the QW slot
+          const int q = pidx[(size_t)l];
+          p->kP_src.push_back(pslot(q, 3));
+          p->lP.push_back(p->kP[l]);
+          p->lQ.push_back(pslot(q, 6));
+        }
       }
     }
     for (int sidx = 0; sidx < L - 1; ++sidx) {
@@ -929,6 +955,10 @@ int orcwa_plan_execute(orcwa_plan* p) {
                             p->dev<const MagMats*>(ct.mag_mats), Pm, Qm, nm);
         zgemm_batched(s, n, n, n, one, (const cd* const*)Pm, 0, 1, n, 0, (const cd* const*)Qm, 0,
                       1, n, 0, zero, A + nq, 0, 1, n, nm);
+        if (p->keep_pq)
+          for (int l = 0; l < p->L; ++l)
+            HIPCHK(hipMemcpyAsync(p->kP[l], p->kP_src[l], nn * sizeof(cd),
+                                  hipMemcpyDeviceToDevice, s));
         copy_batched(s, nn, (const cd* const*)(A + nq), p->dev<cd* const*>(ct.pAorig) + nq, nm);
       }
       EigWorkspace ws;
@@ -1327,9 +1357,9 @@ int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
 namespace orcwa {
 
 int layer_modes(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_instance* inst,
-                const double* layer_cx, const double* layer_cy, LayerModes& out) {
+                const double* layer_cx, const double* layer_cy, LayerModes& out, bool want_pq) {
   orcwa_plan* p = nullptr;
-  int rc = plan_create(ctx, st, inst, 1, true, &p, true, layer_cx, layer_cy);
+  int rc = plan_create(ctx, st, inst, 1, true, &p, true, layer_cx, layer_cy, want_pq);
   if (rc != ORCWA_OK) return rc;
   rc = orcwa_plan_execute(p);
   if (rc == ORCWA_OK) rc = orcwa_plan_sync(p);
@@ -1346,6 +1376,8 @@ int layer_modes(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa_insta
   out.W.assign(p->lW.begin(), p->lW.end());
   out.V.assign(p->lV.begin(), p->lV.end());
   out.keff.assign(p->lK.begin(), p->lK.end());
+  out.P.assign(p->lP.begin(), p->lP.end());
+  out.Q.assign(p->lQ.begin(), p->lQ.end());
   return ORCWA_OK;
 }
 

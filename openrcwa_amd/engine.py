@@ -275,6 +275,63 @@ def rcwa_integrate(ctx: Context, nX, nY, sample, lam, z0, z1, N, kx0=0.0, ky0=0.
     return out
 
 
+def _zresult(n):
+    mats = {k: np.zeros((n, n), dtype=np.complex128)
+            for k in ("Rud", "Tuu", "Rdu", "Tdd", "W0", "V0", "W1", "V1")}
+    vecs = {k: np.zeros(n, dtype=np.complex128) for k in ("gam0", "gam1")}
+    res = _lib.OrcwaZResult()
+    for k, a in list(mats.items()) + list(vecs.items()):
+        setattr(res, k, _dptr(a.view(np.float64)))
+    return res, mats, vecs
+
+
+def _zout(res, mats, vecs):
+    out = {k: a.T.copy() for k, a in mats.items()}    # column-major on the device
+    out.update(vecs)
+    out["status"] = int(res.status)
+    return out
+
+
+def redheffer_integrate(ctx: Context, nX, nY, sample, lam, z0, z1, pe, kx0=0.0, ky0=0.0,
+                        max_segments=0):
+    """The adaptive first-order perturbation integrator
+    (orcwa_gpu_redheffer_integrate; RedhefferIntegrator::compute,
+    core/RedhefferIntegrator.cpp:173-399).  sample(z) -> (coord_x, coord_y, eps)
+    as for rcwa_integrate, called at the points the integrator chooses.
+    Returns the dict of rcwa_integrate plus N, the number of segments."""
+    lib = _lib.load()
+    cx0, cy0, _ = sample(z0)
+    ncx, ncy = len(cx0), len(cy0)
+    failure = []
+
+    def cb(user, z, px, py, pe_):
+        try:
+            cx, cy, eps = sample(z)
+            np.ctypeslib.as_array(px, shape=(ncx,))[:] = np.asarray(cx, dtype=np.float64)
+            np.ctypeslib.as_array(py, shape=(ncy,))[:] = np.asarray(cy, dtype=np.float64)
+            e = np.asarray(eps, dtype=np.complex128).reshape(ncy - 1, ncx - 1)
+            np.ctypeslib.as_array(pe_, shape=(2 * e.size,))[:] = e.view(np.float64).ravel()
+            return 0
+        except Exception as exc:           # reported after the call returns
+            failure.append(exc)
+            return 1
+    fn = _lib.SAMPLER_FN(cb)
+    zs = _lib.OrcwaZSampler(int(nX), int(nY), ncx, ncy, float(lam), float(kx0), float(ky0),
+                            float(z0), float(z1), fn, None)
+    n = 2 * (2 * int(nX) + 1) * (2 * int(nY) + 1)
+    res, mats, vecs = _zresult(n)
+    nseg = ctypes.c_int(0)
+    rc = lib.orcwa_gpu_redheffer_integrate(ctx.handle, ctypes.byref(zs), float(pe),
+                                           int(max_segments), ctypes.byref(res),
+                                           ctypes.byref(nseg))
+    if failure:
+        raise failure[0]
+    _lib.check(rc, "orcwa_gpu_redheffer_integrate")
+    out = _zout(res, mats, vecs)
+    out["N"] = int(nseg.value)
+    return out
+
+
 def _cplx(a, shape):
     a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.complex128), shape))
     return a

@@ -784,6 +784,108 @@ def rcwa_integrator(sample, lam, nX, nY, z0, z1, N, kx0=0.0, ky0=0.0):
                 gam1=g1)
 
 
+def _perturbation(Pr, Qr, W, V, gam, z0, z1, k0, P0, Pm, P1, Q0, Qm, Q1):
+    """RedhefferIntegrator::evaluatePerturbationAndDifference
+    (core/RedhefferIntegrator.cpp:317-399): first-order S-matrix of [z0, z1]
+    around the reference modes (W, V, gam) of (Pr, Qr), the z-integral by
+    Simpson's rule over the samples at z0, the middle and z1.  Returns
+    (tdd, tuu, rud, rdu, diffError)."""
+    luW, luV = sla.lu_factor(W), sla.lu_factor(V)
+
+    def dAB(P, Q):
+        a = sla.lu_solve(luW, (P - Pr) @ V)
+        b = sla.lu_solve(luV, (Q - Qr) @ W)
+        return a + b, a - b
+    dA0, dB0 = dAB(P0, Q0)
+    dA1, dB1 = dAB(Pm, Qm)
+    dA2, dB2 = dAB(P1, Q1)
+    ph = np.exp(1j * 0.5 * (z1 - z0) / k0 * gam)
+    pf = np.exp(1j * (z1 - z0) / k0 * gam)
+    c = 1j * (z1 - z0) / (6 * 2 * k0)
+    tuu = c * (pf[:, None] * dA0 + 4.0 * ph[:, None] * dA1 * ph[None, :] + dA2 * pf[None, :])
+    rud = -c * (pf[:, None] * dB0 * pf[None, :] + 4.0 * ph[:, None] * dB1 * ph[None, :] + dB2)
+    rdu = -c * (dB0 + 4.0 * ph[:, None] * dB1 * ph[None, :] + pf[:, None] * dB2 * pf[None, :])
+    tdd = c * (dA0 * pf[None, :] + 4.0 * ph[:, None] * dA1 * ph[None, :] + pf[:, None] * dA2)
+    err = max(np.abs(tuu).max(), np.abs(rud).max(), np.abs(rdu).max(), np.abs(tdd).max())
+    tuu = tuu + np.diag(pf)
+    tdd = tdd + np.diag(pf)
+    return tdd, tuu, rud, rdu, err
+
+
+def redheffer_integrator(sample, lam, nX, nY, z0, z1, pe, kx0=0.0, ky0=0.0, max_segments=None):
+    """RedhefferIntegrator::compute (core/RedhefferIntegrator.cpp:173-314, the
+    active triple-division branch): first-order perturbation around the
+    segment's midpoint modes; a segment whose perturbation terms exceed pe is
+    cut in three (the middle third keeps the parent's reference); accepted
+    segments, from z0 upwards, are taken from the previous segment's basis
+    into their own (:242-262) and put on top by the Redheffer star
+    (:264-277); finally the top side goes into the basis at z1 (:198-213).
+    Returns the integrator's dict plus N (segments) and nEigVal."""
+    k0 = 2 * PI / lam
+
+    def pq(z):
+        return sampler_pq(sample(z), lam, nX, nY, kx0, ky0)
+    P0, Q0 = pq(z0)
+    W0, V0, g0 = wave_equation(P0, Q0)
+    P1, Q1 = pq(z1)
+    W1, V1, g1 = wave_equation(P1, Q1)
+    n = W0.shape[0]
+    I = np.eye(n)
+    st = dict(Tdd=I.astype(complex), Tuu=I.astype(complex), Rud=np.zeros((n, n), complex),
+              Rdu=np.zeros((n, n), complex), leftW=W0, leftV=V0, N=0, nEig=2)
+    zm = 0.5 * (z0 + z1)
+    Pm, Qm = pq(zm)
+    Wm, Vm, gm = wave_equation(Pm, Qm)
+
+    def accept(W, V, t):
+        tdd, tuu, rud, rdu = t
+        F = np.linalg.solve(W, st["leftW"])
+        G = np.linalg.solve(V, st["leftV"])
+        A, B = 0.5 * (F + G), 0.5 * (F - G)
+        lu = sla.lu_factor(A - rdu @ B)
+        rdu = sla.lu_solve(lu, rdu @ A - B)
+        tdd = sla.lu_solve(lu, tdd)
+        rud = tuu @ B @ tdd + rud
+        tuu = tuu @ A + tuu @ B @ rdu
+        st["leftW"], st["leftV"] = W, V
+        st["Rud"], st["Tuu"], st["Rdu"], st["Tdd"] = redheffer_star_general(
+            (rud, tuu, rdu, tdd), (st["Rud"], st["Tuu"], st["Rdu"], st["Tdd"]))
+
+    def rec(Pr, Qr, W, V, gam, a, b, Pa, Pmid, Pb, Qa, Qmid, Qb):
+        tdd, tuu, rud, rdu, err = _perturbation(Pr, Qr, W, V, gam, a, b, k0,
+                                                Pa, Pmid, Pb, Qa, Qmid, Qb)
+        if err <= pe:
+            st["N"] += 1
+            if max_segments is not None and st["N"] > max_segments:
+                raise RuntimeError("too many segments")
+            accept(W, V, (tdd, tuu, rud, rdu))
+            return
+        m0 = 2.0 / 3.0 * a + 1.0 / 3.0 * b
+        m1 = 1.0 / 3.0 * a + 2.0 / 3.0 * b
+        Pt, Qt = pq(0.5 * (a + m0))
+        Pk, Qk = pq(m0)
+        Wt, Vt, gt = wave_equation(Pt, Qt)
+        rec(Pt, Qt, Wt, Vt, gt, a, m0, Pa, Pt, Pk, Qa, Qt, Qk)
+        Pt, Qt = pq(m1)
+        rec(Pr, Qr, W, V, gam, m0, m1, Pk, Pmid, Pt, Qk, Qmid, Qt)
+        Pk, Qk = pq(0.5 * (m1 + b))
+        Wt, Vt, gt = wave_equation(Pk, Qk)
+        rec(Pk, Qk, Wt, Vt, gt, m1, b, Pt, Pk, Pb, Qt, Qk, Qb)
+
+    rec(Pm, Qm, Wm, Vm, gm, z0, z1, P0, Pm, P1, Q0, Qm, Q1)
+    # the top side into the basis at z1
+    F = np.linalg.solve(st["leftW"], W1)
+    G = np.linalg.solve(st["leftV"], V1)
+    A, B = 0.5 * (F + G), 0.5 * (F - G)
+    lu = sla.lu_factor(A - st["Rud"] @ B)
+    Tuu = sla.lu_solve(lu, st["Tuu"])
+    Rud = sla.lu_solve(lu, st["Rud"] @ A - B)
+    Rdu = st["Rdu"] + st["Tdd"] @ B @ Tuu
+    Tdd = st["Tdd"] @ (B @ Rud + A)
+    return dict(Rud=Rud, Tuu=Tuu, Rdu=Rdu, Tdd=Tdd, W0=W0, V0=V0, gam0=g0, W1=W1, V1=V1,
+                gam1=g1, N=st["N"], nEigVal=st["nEig"])
+
+
 def integrator_harmonic_operators(r):
     """The integrator's S-matrix in harmonic (E-field) space, independent of
     the mode bases: T_down = W0 Tdd W1^-1, R_top = W1 Rud W1^-1,

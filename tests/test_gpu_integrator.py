@@ -71,3 +71,38 @@ def test_staircase_equals_stack_solve(gpu_ctx):
         R, T = rt_from_operators(T_down, R_top, kx, ky, k0, 1.0, 2.1, px, py, i0)
         assert np.abs(R - res.ref[0, q]).max() < 1e-10
         assert np.abs(T - res.trn[0, q]).max() < 1e-10
+
+
+def test_redheffer_uniform_in_z(gpu_ctx):
+    r = engine.redheffer_integrate(gpu_ctx, NHX, NHY, waveguide(False), LAM, Z0, Z1, 1e-3)
+    assert r["status"] == 0 and r["N"] == 1
+    E = np.diag(np.exp(1j * r["gam0"] * (Z1 - Z0) * LAM / (2 * math.pi)))
+    assert np.abs(r["Tuu"] - E).max() < 1e-10 and np.abs(r["Tdd"] - E).max() < 1e-10
+    assert np.abs(r["Rud"]).max() < 1e-10 and np.abs(r["Rdu"]).max() < 1e-10
+
+
+@pytest.mark.parametrize("pe", [1e-2, 3e-3])
+def test_redheffer_against_oracle(gpu_ctx, pe):
+    g = engine.redheffer_integrate(gpu_ctx, NHX, NHY, waveguide(True), LAM, Z0, Z1, pe)
+    ref = o.redheffer_integrator(as_layer(waveguide(True)), LAM, NHX, NHY, Z0, Z1, pe)
+    assert g["status"] == 0 and g["N"] == ref["N"] > 1
+    for a, b in zip(harmonic(g), harmonic(ref)):
+        assert np.abs(a - b).max() < 1e-9 * np.abs(b).max()
+
+
+def test_redheffer_agrees_with_fine_staircase(gpu_ctx):
+    # tests/test_core_integrators.cpp:143-146
+    fine = harmonic(engine.rcwa_integrate(gpu_ctx, NHX, NHY, waveguide(True), LAM, Z0, Z1, 256))
+    g = engine.redheffer_integrate(gpu_ctx, NHX, NHY, waveguide(True), LAM, Z0, Z1, 1e-3)
+    T = harmonic(g)
+    assert np.abs(T[0] - fine[0]).max() / np.abs(fine[0]).max() < 1e-3
+    assert np.abs(T[2] - fine[2]).max() / np.abs(fine[2]).max() < 1e-3
+
+
+def test_redheffer_sampler_error_is_raised(gpu_ctx):
+    def bad(z):
+        if z > 0.5:
+            raise ValueError("outside the sampled range")
+        return waveguide(True)(z)
+    with pytest.raises(ValueError):
+        engine.redheffer_integrate(gpu_ctx, NHX, NHY, bad, LAM, Z0, Z1, 1e-3)

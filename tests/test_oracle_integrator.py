@@ -114,3 +114,25 @@ def test_staircase_equals_rtcm():
     for (px, py), Rr, Tr in zip(o.polarisation_basis(theta, phi), REF, TRN):
         R, T = rt_from_operators(T_down, R_top, s.Kx, s.Ky, k0, 1.0, 2.1, px, py, i0)
         assert np.abs(R - Rr).max() < 1e-12 and np.abs(T - Tr).max() < 1e-12
+
+
+# ---- the adaptive perturbation integrator (core/RedhefferIntegrator.cpp) ----
+def test_redheffer_uniform_in_z_is_one_exact_segment():
+    r = o.redheffer_integrator(as_layer(waveguide(False)), LAM, NHX, NHY, Z0, Z1, 1e-3)
+    assert r["N"] == 1
+    E = np.diag(np.exp(1j * r["gam0"] * (Z1 - Z0) * LAM / (2 * math.pi)))
+    assert np.abs(r["Tuu"] - E).max() < 1e-12 and np.abs(r["Tdd"] - E).max() < 1e-12
+    assert np.abs(r["Rud"]).max() < 1e-12 and np.abs(r["Rdu"]).max() < 1e-12
+
+
+def test_redheffer_agrees_with_fine_staircase():
+    # tests/test_core_integrators.cpp:143-146: pe = 1e-3 within 1e-3 of a fine
+    # RCWAIntegrator (both in the same bases: the modes at z0 and z1)
+    ref = o.rcwa_integrator(as_layer(waveguide(True)), LAM, NHX, NHY, Z0, Z1, 128)
+    scale = np.abs(ref["Tuu"]).max()
+    errs = {}
+    for pe in (1e-2, 1e-3):
+        r = o.redheffer_integrator(as_layer(waveguide(True)), LAM, NHX, NHY, Z0, Z1, pe)
+        errs[pe] = np.abs(r["Tuu"] - ref["Tuu"]).max() / scale
+        assert r["N"] > 1
+    assert errs[1e-3] < 1e-3 and errs[1e-3] < errs[1e-2]

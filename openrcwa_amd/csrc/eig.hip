@@ -269,11 +269,14 @@ __global__ __launch_bounds__(256) void k_hess_panel(int n, int i0, int nb, cd* c
       ai = wave_sum(ai);
       if (lane == 0) s_t[p] = mk(ar, ai);
     }
-    // y = A(r0:n, c+1:n) v(c+1:n): block-start columns (thread per row)
-    const int ra = r0 + tid, rb = r0 + tid + 256;
-    cd ya = mk(0.0, 0.0), yb = mk(0.0, 0.0);
-    if (ra < n) {
+    __syncthreads();
+    // y = A(r0:n, c+1:n) v(c+1:n) over the block-start columns, then
+    // Y(:, j) = tau (y - Y(:, 0:j) t): two rows per thread, 512 rows a pass
+    for (int base = r0; base < n; base += 512) {
+      const int ra = base + tid, rb = base + tid + 256;
+      if (ra >= n) break;
       const bool hb = rb < n;
+      cd ya = mk(0.0, 0.0), yb = mk(0.0, 0.0);
 #pragma unroll 4
       for (int q = c + 1; q < n; ++q) {
         const cd vq = sv[q];
@@ -281,18 +284,14 @@ __global__ __launch_bounds__(256) void k_hess_panel(int n, int i0, int nb, cd* c
         ya += col[ra] * vq;
         if (hb) yb += col[rb] * vq;
       }
-    }
-    __syncthreads();
-    // Y(:, j) = tau (y - Y(:, 0:j) t)
-    if (ra < n) {
       cd acc = ya;
       for (int p = 0; p < j; ++p) acc -= Y[ra + (long long)p * n] * s_t[p];
       Y[ra + (long long)j * n] = t * acc;
-    }
-    if (rb < n) {
-      cd acc = yb;
-      for (int p = 0; p < j; ++p) acc -= Y[rb + (long long)p * n] * s_t[p];
-      Y[rb + (long long)j * n] = t * acc;
+      if (hb) {
+        acc = yb;
+        for (int p = 0; p < j; ++p) acc -= Y[rb + (long long)p * n] * s_t[p];
+        Y[rb + (long long)j * n] = t * acc;
+      }
     }
     // T(0:j, j) = -tau T(0:j, 0:j) t,  T(j, j) = tau
     if (tid < j) {

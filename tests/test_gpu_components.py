@@ -109,8 +109,9 @@ def rcwa_pq(N, seed):
     return P @ Q
 
 
-@pytest.mark.parametrize("N", [1, 3, 7])
+@pytest.mark.parametrize("N", [1, 3, 7, 9])
 def test_zgeev_rcwa_matrices(gpu_ctx, N):
+    # (N = 9: n = 722, Hessenberg panels taller than 512 rows)
     A = np.stack([rcwa_pq(N, s) for s in range(3)])
     lam, W, resid, info = g.zgeev(gpu_ctx, A)
     assert not info.any()
@@ -142,3 +143,13 @@ def test_convolution_matrices(gpu_ctx, nX, nY, xe, ye):
     ref = o.convolution_matrices(xe, ye, eps, 2 * nX + 1, 2 * nY + 1)
     for a, b in zip(got, ref):
         assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+
+
+def test_zgeev_largest_rcwa_matrix(gpu_ctx):
+    # n = 1922 (31 x 31 orders): eigenvalues against LAPACK and the residual
+    A = rcwa_pq(15, 0)[None]
+    lam, W, resid, info = g.zgeev(gpu_ctx, A)
+    assert not info.any()
+    ref = np.linalg.eigvals(A[0])
+    assert match_eigenvalues(lam[0], ref) < 1e-10 * np.abs(ref).max()
+    assert resid[0] < 1e-11

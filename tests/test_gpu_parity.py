@@ -150,3 +150,38 @@ def test_pillar_full_size_properties(gpu_ctx):
     tot = res.R + res.T
     assert (res.ref > -1e-12).all() and (res.trn > -1e-12).all()
     assert (tot < 1 + 1e-9).all() and (tot > 0.3).all()
+
+
+C4_CIRCLE = """OpenRCWA 4 2
+title = circles at the largest order count
+xmesh = -2.5e-07 10 2.5e-07
+ymesh = -2.5e-07 10 2.5e-07
+zmesh = -5e-07 10 -2e-07 10 0.0 10 5e-07
+material = 1 6.25 0 1 0
+geometry = 2 11 -1.5e-07 1.5e-07 -1.5e-07 1.5e-07 -2e-07 0
+planewave = 0 0 {pol}
+pbc = 1 1 0
+rcwaorder = 15 15
+wavelength = 0.55 0.7 1
+end
+"""
+
+
+def test_circle_largest_order_count_properties(gpu_ctx, tmp_path):
+    """A lossless staircase cylinder at 31 x 31 orders (n = 1922, the largest
+    configuration of the survey) through the .orcwa path: energy
+    conservation, TE = TM at normal incidence on the C4-symmetric lattice, and
+    mirror symmetry of the orders."""
+    from openrcwa_amd import orcwa as oc
+    out = {}
+    for pol in (1, 2):
+        p = tmp_path / f"c{pol}.orcwa"
+        p.write_text(C4_CIRCLE.format(pol=pol))
+        out[pol] = oc.run_rcwa(oc.parse_rcwa_input(str(p)), gpu_ctx)
+    for r1, r2 in zip(out[1], out[2]):
+        assert r1.status == 0 and r2.status == 0
+        for r in (r1, r2):
+            assert abs(r.R + r.T - 1.0) < 1e-9
+            g = r.REF_orders.reshape(31, 31)
+            assert np.abs(g - g[::-1, :]).max() < 1e-9 and np.abs(g - g[:, ::-1]).max() < 1e-9
+        assert abs(r1.R - r2.R) < 1e-9 and abs(r1.T - r2.T) < 1e-9

@@ -259,6 +259,9 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   if (!st->thickness) return fail("null thickness array");
   const int nx = 2 * st->nx_harm + 1, ny = 2 * st->ny_harm + 1;
   if (nx > 63 || ny > 63) return fail("at most 31 harmonics per axis are supported");
+  // the cascade's column chain holds 4 n-vectors in LDS (160 KB per workgroup)
+  if (4 * (size_t)(2 * nx * ny) * sizeof(cd) > 160 * 1024)
+    return fail("at most 1280 harmonics ((2Nx+1)(2Ny+1)) are supported");
   if (!layer_cx) {     // (per-layer edges are checked layer by layer below)
     for (int i = 1; i < st->n_cx; ++i)
       if (!(st->coord_x[i] > st->coord_x[i - 1])) return fail("coord_x must be strictly increasing");

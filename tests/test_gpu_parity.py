@@ -185,3 +185,12 @@ def test_circle_largest_order_count_properties(gpu_ctx, tmp_path):
             g = r.REF_orders.reshape(31, 31)
             assert np.abs(g - g[::-1, :]).max() < 1e-9 and np.abs(g - g[:, ::-1]).max() < 1e-9
         assert abs(r1.R - r2.R) < 1e-9 and abs(r1.T - r2.T) < 1e-9
+
+
+def test_too_many_harmonics_is_an_error(gpu_ctx):
+    # 37 x 37 orders (n = 2738) exceed what the cascade keeps in LDS
+    from openrcwa_amd import _lib
+    eps = np.array([[[1.0]], [[2.25]], [[1.0]]], dtype=complex)
+    with pytest.raises(_lib.EngineError):
+        engine.solve_batch(gpu_ctx, 18, 18, [0, 0.5], [0, 0.5], [0.0, 0.1, 0.0], [0.6], [0.0],
+                           [0.0], eps, np.array([[[1.0, 0.0]]], dtype=complex))

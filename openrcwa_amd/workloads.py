@@ -12,6 +12,14 @@ is 256 frequencies (4.3e14 .. 7.5e14 Hz, linear) x 8 incidence directions
 eps 4.0|1.0 at fill 0.5, 200 nm / eps 2.25, normal incidence, 16 frequencies
 4.3e14 .. 7.5e14 Hz.
 
+``large2d`` is the large 2D grating: Nx = Ny = 15 (31 x 31 = 961 orders,
+1922 x 1922 layer matrices), periods 400 x 400 nm, air / six patterned layers /
+eps 2.25.  Each layer holds one inclusion of eps U(2, 12) + i U(0, 0.05) in
+air, a rectangle or a circle; a circle is the 8 x 8 staircase of cells of
+its bounding box whose centres lie inside it (the curved-shape rule of the
+.orcwa input path).  Thickness U(50, 300) nm; normal incidence, 64
+frequencies 4.3e14 .. 7.5e14 Hz.
+
 Random numbers: splitmix64, 53-bit uniforms in [0, 1).
 """
 from __future__ import annotations
@@ -95,6 +103,60 @@ def pillar2d(n_freq=256, thetas=(0.0, 35.0 / 3.0, 70.0 / 3.0, 35.0), phis=(0.0, 
     return dict(nX=nX, nY=nY, coord_x=np.array(xe), coord_y=np.array(ye), thickness=thick,
                 eps=eps, lam=lam, kx0=kx0, ky0=ky0, pols=pols, angles=ang, layers=layers,
                 period=period, rayleigh=rayleigh, freqs=freqs)
+
+
+LARGE_SEED = 20261203
+
+
+def large2d(n_freq=64, n_layers=6, nX=15, nY=15, seed=LARGE_SEED, period=400e-9):
+    """The large 2D grating as batch arrays (one instance per frequency, normal
+    incidence, TE and TM).  Lengths in metres."""
+    rng = SplitMix64(seed)
+    shapes = []
+    for _ in range(n_layers):
+        circle = rng.uniform() < 0.5
+        e1 = complex(rng.uniform(2.0, 12.0), rng.uniform(0.0, 0.05))
+        d = rng.uniform(50e-9, 300e-9)
+        if circle:
+            r = rng.uniform(0.15, 0.4) * period
+            cx = cy = 0.5 * period
+            ex = list(np.linspace(cx - r, cx + r, 9))
+            ey = list(np.linspace(cy - r, cy + r, 9))
+        else:
+            w, h = rng.uniform(0.2, 0.8) * period, rng.uniform(0.2, 0.8) * period
+            x0, y0 = rng.uniform(0.0, period - w), rng.uniform(0.0, period - h)
+            ex, ey, r, cx, cy = [x0, x0 + w], [y0, y0 + h], 0.0, 0.0, 0.0
+        shapes.append(dict(circle=circle, eps1=e1, thickness=d, ex=ex, ey=ey, r=r, cx=cx, cy=cy))
+    xe = sorted({0.0, period, *[x for sh in shapes for x in sh["ex"]]})
+    ye = sorted({0.0, period, *[y for sh in shapes for y in sh["ey"]]})
+    xc = [0.5 * (xe[i] + xe[i + 1]) for i in range(len(xe) - 1)]
+    yc = [0.5 * (ye[j] + ye[j + 1]) for j in range(len(ye) - 1)]
+
+    def grid(sh):
+        g = np.ones((len(yc), len(xc)), dtype=complex)
+        for j, y in enumerate(yc):
+            for i, x in enumerate(xc):
+                inside = sh["ex"][0] < x < sh["ex"][-1] and sh["ey"][0] < y < sh["ey"][-1]
+                if inside and sh["circle"]:
+                    # the staircase cell holding (x, y) is in when its centre is
+                    ix = int(np.searchsorted(sh["ex"], x)) - 1
+                    iy = int(np.searchsorted(sh["ey"], y)) - 1
+                    mx = 0.5 * (sh["ex"][ix] + sh["ex"][ix + 1])
+                    my = 0.5 * (sh["ey"][iy] + sh["ey"][iy + 1])
+                    inside = (mx - sh["cx"]) ** 2 + (my - sh["cy"]) ** 2 <= sh["r"] ** 2
+                if inside:
+                    g[j, i] = sh["eps1"]
+        return g
+    eps = np.stack([np.ones((len(yc), len(xc)), dtype=complex)] + [grid(sh) for sh in shapes] +
+                   [np.full((len(yc), len(xc)), 2.25 + 0j)])
+    thick = np.array([0.0] + [sh["thickness"] for sh in shapes] + [0.0])
+    freqs = np.linspace(4.3e14, 7.5e14, n_freq)
+    lam = C0 / freqs
+    te, tm = polarisation_basis(0.0, 0.0)
+    pols = np.array([[[te[0], te[1]], [tm[0], tm[1]]]] * n_freq, dtype=complex)
+    return dict(nX=nX, nY=nY, coord_x=np.array(xe), coord_y=np.array(ye), thickness=thick,
+                eps=eps, lam=lam, kx0=np.zeros(n_freq), ky0=np.zeros(n_freq), pols=pols,
+                shapes=shapes, period=period, freqs=freqs)
 
 
 def grating1d(nX=5, n_freq=16):

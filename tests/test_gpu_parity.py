@@ -194,3 +194,19 @@ def test_too_many_harmonics_is_an_error(gpu_ctx):
     with pytest.raises(_lib.EngineError):
         engine.solve_batch(gpu_ctx, 18, 18, [0, 0.5], [0, 0.5], [0.0, 0.1, 0.0], [0.6], [0.0],
                            [0.0], eps, np.array([[[1.0, 0.0]]], dtype=complex))
+
+
+def test_large2d_against_oracle(gpu_ctx):
+    """The large 2D grating (31 x 31 orders, n = 1922) with a rectangle and a
+    staircase circle layer, per order against the FP64 oracle."""
+    w = workloads.large2d(n_freq=2, n_layers=2)
+    res = engine.solve_batch(gpu_ctx, w["nX"], w["nY"], w["coord_x"], w["coord_y"],
+                             w["thickness"], w["lam"], w["kx0"], w["ky0"], w["eps"], w["pols"])
+    assert not res.status.any()
+    REF, TRN, _ = o.solve_stack(w["nX"], w["nY"], w["period"], w["period"], w["coord_x"],
+                                w["coord_y"], list(w["eps"]), list(w["thickness"]), w["lam"][0],
+                                0.0, 0.0, pols=[tuple(p) for p in w["pols"][0]])
+    for q in range(2):
+        assert np.abs(res.ref[0, q] - REF[q]).max() < 1e-9
+        assert np.abs(res.trn[0, q] - TRN[q]).max() < 1e-9
+    assert (res.R + res.T <= 1 + 1e-9).all()

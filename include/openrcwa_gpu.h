@@ -87,6 +87,14 @@ typedef struct {
                                 magnetic P and Q of rcwa/Layer.cpp:129-145, 150-164,
                                 206-213; the half-spaces must stay homogeneous in eps
                                 and mu. */
+  /* NULL (the stack's edges) or this instance's n_cx / n_cy edges: same counts
+     and periods as the stack, sorted, a repeated edge being an empty cell.  Lets
+     stacks of one order count and layer count but different cell layouts share
+     a batch (RCWASolver::addLayer takes each stack's own edges,
+     rcwa/RCWASolver.cpp:108-137). */
+  const double *coord_x;
+  const double *coord_y;
+  const double *thickness;   /* NULL (the stack's) or this instance's [L] thicknesses */
 } orcwa_instance;
 
 /* Output of one instance.  Orders k = (2Ny+1)(m+Nx) + (n+Ny), h = (2Nx+1)(2Ny+1)

@@ -31,7 +31,8 @@ class OrcwaStack(ctypes.Structure):
 class OrcwaInstance(ctypes.Structure):
     _fields_ = [("lam", c_double), ("kx0", c_double), ("ky0", c_double),
                 ("eps", POINTER(c_double)), ("n_pol", c_int), ("pol", POINTER(c_double)),
-                ("mu", POINTER(c_double))]
+                ("mu", POINTER(c_double)), ("coord_x", POINTER(c_double)),
+                ("coord_y", POINTER(c_double)), ("thickness", POINTER(c_double))]
 
 
 class OrcwaResult(ctypes.Structure):

@@ -247,6 +247,9 @@ extern "C" int rcwa_run(FILE* fp_log) {
     inst[f].n_pol = 2;
     inst[f].pol = pp;
     inst[f].mu = nullptr;
+    inst[f].coord_x = nullptr;
+    inst[f].coord_y = nullptr;
+    inst[f].thickness = nullptr;
     res[f].ref = &ref[(size_t)f * 2 * nOrd];
     res[f].trn = &trn[(size_t)f * 2 * nOrd];
     res[f].amp = nullptr;

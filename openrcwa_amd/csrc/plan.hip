@@ -82,6 +82,7 @@ struct orcwa_plan {
   int ncx = 0, ncy = 0, ncell = 0, maxpol = 1;
   double Lx = 0, Ly = 0;
   std::vector<double> cx, cy, thick;
+  std::vector<double> ithick;      // [count][L]: each instance's thicknesses
   std::vector<double> lam, kx0, ky0;
   std::vector<int> npol;
   std::vector<cd> pol;             // [count][8][2]
@@ -262,12 +263,21 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   // the cascade's column chain holds 4 n-vectors in LDS (160 KB per workgroup)
   if (4 * (size_t)(2 * nx * ny) * sizeof(cd) > 160 * 1024)
     return fail("at most 1280 harmonics ((2Nx+1)(2Ny+1)) are supported");
-  if (!layer_cx) {     // (per-layer edges are checked layer by layer below)
-    for (int i = 1; i < st->n_cx; ++i)
-      if (!(st->coord_x[i] > st->coord_x[i - 1])) return fail("coord_x must be strictly increasing");
-    for (int i = 1; i < st->n_cy; ++i)
-      if (!(st->coord_y[i] > st->coord_y[i - 1])) return fail("coord_y must be strictly increasing");
-  }
+  bool own_edges = (layer_cx != nullptr);
+  for (int i = 0; i < count; ++i)
+    if (inst[i].coord_x || inst[i].coord_y) own_edges = true;
+  // shared edges are strictly increasing; per-layer or per-instance edges are
+  // checked one by one below (they may repeat: an empty cell)
+  for (int i = 1; i < st->n_cx; ++i)
+    if (own_edges ? !(st->coord_x[i] >= st->coord_x[i - 1])
+                  : !(st->coord_x[i] > st->coord_x[i - 1]))
+      return fail("coord_x must be strictly increasing");
+  for (int i = 1; i < st->n_cy; ++i)
+    if (own_edges ? !(st->coord_y[i] >= st->coord_y[i - 1])
+                  : !(st->coord_y[i] > st->coord_y[i - 1]))
+      return fail("coord_y must be strictly increasing");
+  if (!(st->coord_x[st->n_cx - 1] > st->coord_x[0]) || !(st->coord_y[st->n_cy - 1] > st->coord_y[0]))
+    return fail("the periods must be positive");
 
   if (keep && count != 1) return fail("a field plan holds one instance");
   if ((layer_cx || layer_cy || modes_only) && count != 1)
@@ -278,7 +288,6 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->keep = keep;
   p->modes_only = modes_only;
   p->keep_pq = keep_pq && modes_only && keep;
-  p->per_grid_edges = (layer_cx != nullptr);
   p->Nx = st->nx_harm;
   p->Ny = st->ny_harm;
   p->nx = nx;
@@ -295,6 +304,10 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->Lx = p->cx.back() - p->cx.front();
   p->Ly = p->cy.back() - p->cy.front();
   p->thick.assign(st->thickness, st->thickness + p->L);
+  for (int i = 0; i < count; ++i) {
+    const double* t = inst[i].thickness ? inst[i].thickness : st->thickness;
+    p->ithick.insert(p->ithick.end(), t, t + p->L);
+  }
   const int L = p->L, h = p->h, n = p->n, ncell = p->ncell;
 
   // ---- instances: classify layers, dedupe grids.  A layer is homogeneous
@@ -316,46 +329,66 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->mu_ref.resize(count);
   p->mu_trn.resize(count);
   // a grid is its values (and, with per-layer edges, the layer's edges)
-  auto grid_of = [&](const double* v, int l) {
+  // instances may bring their own cell edges (same counts and periods)
+  bool inst_edges = false;
+  for (int i = 0; i < count; ++i)
+    if (inst[i].coord_x || inst[i].coord_y) inst_edges = true;
+  if (inst_edges && layer_cx) {
+    delete p;
+    return fail("per-layer and per-instance edges cannot be combined");
+  }
+  p->per_grid_edges = (layer_cx != nullptr) || inst_edges;
+  auto check_edges = [&](const double* x, const double* y) {
+    for (int i = 1; i < p->ncx; ++i)
+      if (!(x[i] >= x[i - 1])) return false;
+    for (int i = 1; i < p->ncy; ++i)
+      if (!(y[i] >= y[i - 1])) return false;
+    return std::fabs((x[p->ncx - 1] - x[0]) - p->Lx) <= 1e-12 * p->Lx &&
+           std::fabs((y[p->ncy - 1] - y[0]) - p->Ly) <= 1e-12 * p->Ly;
+  };
+  // a grid is its values (and, with per-grid edges, its edges)
+  auto grid_of = [&](const double* v, const double* ex, const double* ey) {
     std::vector<double> key(v, v + 2 * ncell);
-    if (layer_cx) {
-      key.insert(key.end(), layer_cx + (size_t)l * p->ncx, layer_cx + (size_t)(l + 1) * p->ncx);
-      key.insert(key.end(), layer_cy + (size_t)l * p->ncy, layer_cy + (size_t)(l + 1) * p->ncy);
+    if (p->per_grid_edges) {
+      key.insert(key.end(), ex, ex + p->ncx);
+      key.insert(key.end(), ey, ey + p->ncy);
     }
     auto it = grid_index.find(key);
     if (it != grid_index.end()) return it->second;
     const int g = (int)grid_index.size();
     grid_index.emplace(std::move(key), g);
     for (int c = 0; c < ncell; ++c) p->grid_vals.push_back(mk(v[2 * c], v[2 * c + 1]));
-    if (layer_cx) {
-      p->grid_cx.insert(p->grid_cx.end(), layer_cx + (size_t)l * p->ncx,
-                        layer_cx + (size_t)(l + 1) * p->ncx);
-      p->grid_cy.insert(p->grid_cy.end(), layer_cy + (size_t)l * p->ncy,
-                        layer_cy + (size_t)(l + 1) * p->ncy);
+    if (p->per_grid_edges) {
+      p->grid_cx.insert(p->grid_cx.end(), ex, ex + p->ncx);
+      p->grid_cy.insert(p->grid_cy.end(), ey, ey + p->ncy);
     }
     return g;
   };
-  if (layer_cx) {
-    if (!layer_cy) {
-      delete p;
-      return fail("per-layer edges need both axes");
-    }
-    for (int l = 0; l < L; ++l) {
-      const double* x = layer_cx + (size_t)l * p->ncx;
-      const double* y = layer_cy + (size_t)l * p->ncy;
-      // (a moving edge may meet its neighbour: an empty cell adds nothing to
-      // the Fourier series)
-      for (int i = 1; i < p->ncx; ++i)
-        if (!(x[i] >= x[i - 1])) { delete p; return fail("layer edges must be sorted"); }
-      for (int i = 1; i < p->ncy; ++i)
-        if (!(y[i] >= y[i - 1])) { delete p; return fail("layer edges must be sorted"); }
-      if (std::fabs((x[p->ncx - 1] - x[0]) - p->Lx) > 1e-12 * p->Lx ||
-          std::fabs((y[p->ncy - 1] - y[0]) - p->Ly) > 1e-12 * p->Ly) {
-        delete p;
-        return fail("every layer must have the same period");
-      }
-    }
+  if (layer_cx && !layer_cy) {
+    delete p;
+    return fail("per-layer edges need both axes");
   }
+  // the edges of (instance i, layer l); (a moving edge may meet its
+  // neighbour: an empty cell adds nothing to the Fourier series)
+  auto ex_of = [&](int i, int l) -> const double* {
+    if (layer_cx) return layer_cx + (size_t)l * p->ncx;
+    return inst[i].coord_x ? inst[i].coord_x : st->coord_x;
+  };
+  auto ey_of = [&](int i, int l) -> const double* {
+    if (layer_cy) return layer_cy + (size_t)l * p->ncy;
+    return inst[i].coord_y ? inst[i].coord_y : st->coord_y;
+  };
+  if (layer_cx)
+    for (int l = 0; l < L; ++l)
+      if (!check_edges(ex_of(0, l), ey_of(0, l))) {
+        delete p;
+        return fail("layer edges must be sorted, with the period of the first layer");
+      }
+  for (int i = 0; i < count && inst_edges; ++i)
+    if (!check_edges(ex_of(i, 0), ey_of(i, 0))) {
+      delete p;
+      return fail("instance edges must be sorted, with the stack's period");
+    }
   auto uniform_of = [&](const double* v) {
     for (int c = 1; c < ncell; ++c)
       if (v[2 * c] != v[0] || v[2 * c + 1] != v[1]) return false;
@@ -398,14 +431,14 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
         // P and Q apart: the magnetic builder with a mu = 1 grid
         std::vector<double> ones(2 * (size_t)ncell, 0.0);
         for (int c = 0; c < ncell; ++c) ones[2 * c] = 1.0;
-        p->gid[il] = grid_of(e, l);
-        p->mgid[il] = grid_of(m ? m : ones.data(), l);
+        p->gid[il] = grid_of(e, ex_of(i, l), ey_of(i, l));
+        p->mgid[il] = grid_of(m ? m : ones.data(), ex_of(i, l), ey_of(i, l));
       } else if (uniform_of(e) && (!m || uniform_of(m))) {
         p->ueps[il] = mk(e[0], e[1]);
         if (m) p->umu[il] = mk(m[0], m[1]);
       } else {
-        p->gid[il] = grid_of(e, l);
-        if (!mu_one) p->mgid[il] = grid_of(m, l);
+        p->gid[il] = grid_of(e, ex_of(i, l), ey_of(i, l));
+        if (!mu_one) p->mgid[il] = grid_of(m, ex_of(i, l), ey_of(i, l));
       }
     }
     if (!modes_only && (p->gid[(size_t)i * L] >= 0 || p->gid[(size_t)i * L + L - 1] >= 0)) {
@@ -782,7 +815,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
       for (int ii = 0; ii < ct.ninst; ++ii) {
         const int i = ct.i0 + ii;
         kb.push_back(Kof(ii, b));
-        k0d.push_back(2.0 * kPi / p->lam[i] * p->thick[b]);
+        k0d.push_back(2.0 * kPi / p->lam[i] * p->ithick[(size_t)i * L + b]);
         const bool a_mag = pidx[(size_t)ii * L + a] >= 0 && p->mgid[(size_t)i * L + a] >= 0;
         ka.push_back(pidx[(size_t)ii * L + a] >= 0 && !a_mag ? Kof(ii, a) : nullptr);
         if (a_mag) {
@@ -1203,7 +1236,8 @@ int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
                           int* cols, double* field, int* status) {
   if (!ctx || !st || !inst || !req || !rows || !cols) return fail("field image: null argument");
   const int L = st->n_layers;
-  if (L < 3 || !st->thickness)
+  const double* thick = inst->thickness ? inst->thickness : st->thickness;
+  if (L < 3 || !thick)
     return fail("field image: the stack needs at least one interior layer (3 layers)");
   if (req->component < 0 || req->component > 5) return fail("field image: unknown component");
   if (req->slice < 0 || req->slice > 2) return fail("field image: unknown slice");
@@ -1217,7 +1251,7 @@ int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
   const double Lx = st->coord_x[st->n_cx - 1] - st->coord_x[0];
   const double Ly = st->coord_y[st->n_cy - 1] - st->coord_y[0];
   double zint = 0.0;
-  for (int l = 1; l < L - 1; ++l) zint += st->thickness[l];
+  for (int l = 1; l < L - 1; ++l) zint += thick[l];
   // image axes (rcwa/RCWASolver.cpp:730-779)
   double s1, l1, s2, l2;
   if (req->slice == 1) {
@@ -1235,7 +1269,7 @@ int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
   auto locate = [&](double z, FieldRow& fr) {
     double z0 = 0.0;
     for (int j = 1; j < L - 1; ++j) {
-      const double z1 = z0 + st->thickness[j];
+      const double z1 = z0 + thick[j];
       if (z0 <= z && z <= z1) {
         fr.layer = j;
         fr.pad = 0;
@@ -1320,7 +1354,7 @@ int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
   HIPCHK(hipMemsetAsync(cm + (size_t)(L - 1) * n, 0, (size_t)n * sizeof(cd), s));
   for (int j = L - 2; j >= 1; --j) {
     const int si = L - 2 - j;    // the cascade step of the interface j | j+1
-    const double k0d = (j + 1 == L - 1) ? 0.0 : k0 * st->thickness[j + 1];
+    const double k0d = (j + 1 == L - 1) ? 0.0 : k0 * thick[j + 1];
     launch_field_up(s, n, p->kA[si], p->kB[si], cp + (size_t)(j + 1) * n, cm + (size_t)(j + 1) * n,
                     p->lK[j + 1], k0d, cm + (size_t)j * n);
   }

@@ -73,7 +73,7 @@ class Plan:
     whole batch on the GPU."""
 
     def __init__(self, ctx: Context, nX: int, nY: int, coord_x, coord_y, thickness,
-                 lam, kx0, ky0, eps, pols, mu=None):
+                 lam, kx0, ky0, eps, pols, mu=None, inst_coords=None, inst_thickness=None):
         self._lib = _lib.load()
         self.ctx = ctx
         self.coord_x = np.ascontiguousarray(coord_x, dtype=np.float64)
@@ -132,6 +132,17 @@ class Plan:
             inst[i].pol = _dptr(pol_flat[i])
             if mu_flat is not None:
                 inst[i].mu = _dptr(mu_flat[0 if mu_flat.shape[0] == 1 else i])
+        if inst_coords is not None:
+            # per-instance cell edges (B, ncx) and (B, ncy)
+            self._icx = np.ascontiguousarray(inst_coords[0], dtype=np.float64)
+            self._icy = np.ascontiguousarray(inst_coords[1], dtype=np.float64)
+            for i in range(B):
+                inst[i].coord_x = _dptr(self._icx[i])
+                inst[i].coord_y = _dptr(self._icy[i])
+        if inst_thickness is not None:
+            self._ith = np.ascontiguousarray(inst_thickness, dtype=np.float64)
+            for i in range(B):
+                inst[i].thickness = _dptr(self._ith[i])
         self._inst = inst
         h = c_void_p()
         _lib.check(self._lib.orcwa_plan_create(ctx.handle, ctypes.byref(st), inst, B,
@@ -183,6 +194,59 @@ def solve_batch(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0
         return p.results()
     finally:
         p.close()
+
+
+def uneven_plans(ctx: Context, stacks):
+    """Plans for independent stacks of different order counts, layer counts,
+    cell layouts and thicknesses (dicts as workloads.uneven_stacks makes
+    them): grouped by (Nx, Ny, layers, periods), each group one batch whose
+    instances carry their own edges (padded to the group's edge count with
+    empty cells) and thicknesses.  Yields (plan, indices) one group at a time
+    (a group's workspace is released when its plan is closed)."""
+    groups = {}
+    for i, s_ in enumerate(stacks):
+        cx, cy = np.asarray(s_["coord_x"]), np.asarray(s_["coord_y"])
+        key = (int(s_["nX"]), int(s_["nY"]), len(s_["thickness"]), float(cx[-1] - cx[0]),
+               float(cy[-1] - cy[0]))
+        groups.setdefault(key, []).append(i)
+    for (nX, nY, L, _, _), idx in groups.items():
+        mx = max(len(stacks[i]["coord_x"]) for i in idx)
+        my = max(len(stacks[i]["coord_y"]) for i in idx)
+        cxs, cys, eps = [], [], []
+        for i in idx:
+            s_ = stacks[i]
+            cx, cy = np.asarray(s_["coord_x"], float), np.asarray(s_["coord_y"], float)
+            e = np.asarray(s_["eps"], complex)
+            # empty cells at the far edges: repeat the last edge and the last
+            # column / row of every grid
+            px, py = mx - cx.size, my - cy.size
+            cxs.append(np.concatenate([cx, np.full(px, cx[-1])]))
+            cys.append(np.concatenate([cy, np.full(py, cy[-1])]))
+            e = np.concatenate([e, np.repeat(e[:, :, -1:], px, axis=2)], axis=2)
+            e = np.concatenate([e, np.repeat(e[:, -1:, :], py, axis=1)], axis=1)
+            eps.append(e)
+        thick = np.array([np.asarray(stacks[i]["thickness"], float) for i in idx])
+        yield (Plan(ctx, nX, nY, cxs[0], cys[0], thick[0],
+                    [stacks[i]["lam"] for i in idx], [stacks[i]["kx0"] for i in idx],
+                    [stacks[i]["ky0"] for i in idx], np.stack(eps),
+                    np.stack([stacks[i]["pol"] for i in idx]),
+                    inst_coords=(np.stack(cxs), np.stack(cys)), inst_thickness=thick), idx)
+
+
+def solve_uneven(ctx: Context, stacks):
+    """uneven_plans + execute + results.  Returns a list of (ref (npol, h),
+    trn (npol, h), status) in the input order."""
+    out = [None] * len(stacks)
+    for p, idx in uneven_plans(ctx, stacks):
+        try:
+            p.execute()
+            p.sync()
+            r = p.results()
+        finally:
+            p.close()
+        for k, i in enumerate(idx):
+            out[i] = (r.ref[k], r.trn[k], int(r.status[k]))
+    return out
 
 
 def field_image(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps, pol,

@@ -12,6 +12,14 @@ is 256 frequencies (4.3e14 .. 7.5e14 Hz, linear) x 8 incidence directions
 eps 4.0|1.0 at fill 0.5, 200 nm / eps 2.25, normal incidence, 16 frequencies
 4.3e14 .. 7.5e14 Hz.
 
+``uneven_stacks`` is the uneven batch: per stack N ~ U{3..12} ((2N+1)^2
+orders), 1 to 10 finite layers between air and eps 2.25, period 400 nm.  A
+layer is a rectangle of eps U(2, 12) + i U(0, 0.1) in air, a rectangle of a
+Lorentz material (eps_inf U(1, 4), 1 to 3 poles ae^2 / (ce^2 - w^2 - i be w)
+with ae, be, ce in rad/s around 1e15), or a homogeneous lossy metal film
+(Re eps U(-30, -5), Im eps U(0.5, 3)); thickness U(20, 300) nm; wavelength
+U(400, 800) nm, theta U(0, 40) deg, phi U(0, 90) deg.
+
 ``large2d`` is the large 2D grating: Nx = Ny = 15 (31 x 31 = 961 orders,
 1922 x 1922 layer matrices), periods 400 x 400 nm, air / six patterned layers /
 eps 2.25.  Each layer holds one inclusion of eps U(2, 12) + i U(0, 0.05) in
@@ -103,6 +111,76 @@ def pillar2d(n_freq=256, thetas=(0.0, 35.0 / 3.0, 70.0 / 3.0, 35.0), phis=(0.0, 
     return dict(nX=nX, nY=nY, coord_x=np.array(xe), coord_y=np.array(ye), thickness=thick,
                 eps=eps, lam=lam, kx0=kx0, ky0=ky0, pols=pols, angles=ang, layers=layers,
                 period=period, rayleigh=rayleigh, freqs=freqs)
+
+
+UNEVEN_SEED = 20261204
+
+
+def lorentz(einf, poles, omega):
+    """eps(w) = einf + sum ae^2 / (ce^2 - w^2 - i be w) (the pole form of
+    sol/rcwa_bridge.cpp:105-123 and rcwa/RCWADriver.cpp:96-123)."""
+    e = complex(einf)
+    for ae, be, ce in poles:
+        e += ae * ae / (ce * ce - omega * omega - 1j * be * omega)
+    return e
+
+
+def uneven_stacks(count=1024, seed=UNEVEN_SEED, n_range=(3, 12), layer_range=(1, 10),
+                  period=400e-9):
+    """A list of independent stacks (dicts with nX, nY, coord_x, coord_y, eps
+    (L, ncy-1, ncx-1), thickness, lam, kx0, ky0, theta, phi, pol (TE, TM)).
+    Lengths in metres."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        N = n_range[0] + int(rng.uniform() * (n_range[1] - n_range[0] + 1))
+        nl = layer_range[0] + int(rng.uniform() * (layer_range[1] - layer_range[0] + 1))
+        lam = rng.uniform(400e-9, 800e-9)
+        omega = 2.0 * math.pi * C0 / lam
+        lay = []
+        for _ in range(nl):
+            kind = int(rng.uniform() * 3)
+            d = rng.uniform(20e-9, 300e-9)
+            if kind == 2:        # homogeneous metal film
+                e = complex(rng.uniform(-30.0, -5.0), rng.uniform(0.5, 3.0))
+                lay.append(dict(rect=None, eps1=e, thickness=d))
+                continue
+            w, h = rng.uniform(0.2, 0.8) * period, rng.uniform(0.2, 0.8) * period
+            x0, y0 = rng.uniform(0.0, period - w), rng.uniform(0.0, period - h)
+            if kind == 0:
+                e = complex(rng.uniform(2.0, 12.0), rng.uniform(0.0, 0.1))
+            else:
+                einf = rng.uniform(1.0, 4.0)
+                npole = 1 + int(rng.uniform() * 3)
+                poles = [(rng.uniform(0.5e15, 2e15), rng.uniform(0.01e15, 0.3e15),
+                          rng.uniform(1e15, 6e15)) for _ in range(npole)]
+                e = lorentz(einf, poles, omega)
+            lay.append(dict(rect=(x0, x0 + w, y0, y0 + h), eps1=e, thickness=d))
+        xe = sorted({0.0, period, *[v for l in lay if l["rect"] for v in l["rect"][:2]]})
+        ye = sorted({0.0, period, *[v for l in lay if l["rect"] for v in l["rect"][2:]]})
+        xc = np.array([0.5 * (xe[i] + xe[i + 1]) for i in range(len(xe) - 1)])
+        yc = np.array([0.5 * (ye[j] + ye[j + 1]) for j in range(len(ye) - 1)])
+        grids = [np.ones((yc.size, xc.size), dtype=complex)]
+        for l in lay:
+            if l["rect"] is None:
+                grids.append(np.full((yc.size, xc.size), l["eps1"], dtype=complex))
+            else:
+                x0, x1, y0, y1 = l["rect"]
+                g = np.ones((yc.size, xc.size), dtype=complex)
+                g[np.ix_((yc > y0) & (yc < y1), (xc > x0) & (xc < x1))] = l["eps1"]
+                grids.append(g)
+        grids.append(np.full((yc.size, xc.size), 2.25 + 0j))
+        th, ph = rng.uniform(0.0, 40.0), rng.uniform(0.0, 90.0)
+        k0 = 2.0 * math.pi / lam
+        t, p = math.radians(th), math.radians(ph)
+        te, tm = polarisation_basis(th, ph)
+        out.append(dict(nX=N, nY=N, period=period, coord_x=np.array(xe), coord_y=np.array(ye),
+                        eps=np.stack(grids),
+                        thickness=np.array([0.0] + [l["thickness"] for l in lay] + [0.0]),
+                        lam=lam, kx0=k0 * math.sin(t) * math.cos(p),
+                        ky0=k0 * math.sin(t) * math.sin(p), theta=th, phi=ph,
+                        pol=np.array([[te[0], te[1]], [tm[0], tm[1]]], dtype=complex)))
+    return out
 
 
 LARGE_SEED = 20261203

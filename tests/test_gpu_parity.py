@@ -210,3 +210,26 @@ def test_large2d_against_oracle(gpu_ctx):
         assert np.abs(res.ref[0, q] - REF[q]).max() < 1e-9
         assert np.abs(res.trn[0, q] - TRN[q]).max() < 1e-9
     assert (res.R + res.T <= 1 + 1e-9).all()
+
+
+def _oracle_stack(s_):
+    return o.solve_stack(s_["nX"], s_["nY"], s_["period"], s_["period"], s_["coord_x"],
+                         s_["coord_y"], list(s_["eps"]), list(s_["thickness"]), s_["lam"],
+                         s_["theta"], s_["phi"], pols=[tuple(p) for p in s_["pol"]])
+
+
+def test_uneven_stacks_against_oracle(gpu_ctx):
+    """The uneven batch: stacks of different order counts, layer counts, cell
+    layouts and thicknesses, with Lorentz-dispersive and metal-film layers,
+    grouped into batches whose instances carry their own edges and
+    thicknesses (one group holds six stacks of one shape)."""
+    stacks = (workloads.uneven_stacks(8, n_range=(3, 6), layer_range=(1, 5)) +
+              workloads.uneven_stacks(6, seed=7, n_range=(4, 4), layer_range=(3, 3)) +
+              workloads.uneven_stacks(1, seed=11, n_range=(10, 10), layer_range=(2, 2)))
+    res = engine.solve_uneven(gpu_ctx, stacks)
+    for s_, (ref, trn, st) in zip(stacks, res):
+        assert st == 0
+        REF, TRN, _ = _oracle_stack(s_)
+        for q in range(2):
+            assert np.abs(ref[q] - REF[q]).max() < 1e-9
+            assert np.abs(trn[q] - TRN[q]).max() < 1e-9

@@ -13,8 +13,9 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // GEMM: 64x64 output tile per workgroup (4 waves, 32x32 each = 2x2 MFMA
-// 16x16 sub-tiles), K staged through LDS 16 at a time, re/im split into
-// separate LDS planes so each MFMA reads one double per lane.  Complex
+// 16x16 sub-tiles), K staged through LDS 16 at a time (32 spills registers
+// and is slower), re/im split into separate LDS planes so each MFMA reads one
+// double per lane.  Complex
 // products in three real MFMAs (Gauss); the launch bound asks for two
 // workgroups per CU, which keeps the 64x64 tile's accumulators in VGPRs
 // without spilling.
@@ -23,12 +24,25 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // TM x TN output tile (TM, TN in {16, 32, 64}): 4 waves in WR x WC, each wave
 // (TM/WR) x (TN/WC) = BI x BJ MFMA 16 x 16 blocks.
+#ifndef GEMM_KS
+#define GEMM_KS 16
+#endif
+#ifndef GEMM_PAD
+#define GEMM_PAD 1
+#endif
+#ifndef ORCWA_GEMM_XCD
+#define ORCWA_GEMM_XCD 1
+#endif
+#ifndef ORCWA_GEMM_XCD_MIN
+#define ORCWA_GEMM_XCD_MIN 64
+#endif
+
 template <int TM, int TN>
 struct TileShape {
   static constexpr int WR = (TM >= 32 && TN >= 32) ? 2 : (TN == 16 ? 4 : 1);
   static constexpr int WC = 4 / WR;
   static constexpr int BI = TM / (16 * WR), BJ = TN / (16 * WC);
-  static constexpr int LA = TM / 16, LB = TN / 16;   // loads per thread per K step of 16
+  static constexpr int LA = TM * GEMM_KS / 256, LB = TN * GEMM_KS / 256;  // loads per thread per K step
   static_assert(BI >= 1 && BJ >= 1 && WR * WC == 4, "tile shape");
 };
 
@@ -40,16 +54,40 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
     cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
   using TS = TileShape<TM, TN>;
   constexpr int BI = TS::BI, BJ = TS::BJ, LA = TS::LA, LB = TS::LB, WC = TS::WC;
-  __shared__ double sAr[16][TM];
-  __shared__ double sAi[16][TM];
-  __shared__ double sBr[16][TN];
-  __shared__ double sBi[16][TN];
+  // rows padded by one double: the k-fast stores (16 lanes down one column)
+  // then fall in distinct banks
+  __shared__ double sAr[GEMM_KS][TM + GEMM_PAD];
+  __shared__ double sAi[GEMM_KS][TM + GEMM_PAD];
+  __shared__ double sBr[GEMM_KS][TN + GEMM_PAD];
+  __shared__ double sBi[GEMM_KS][TN + GEMM_PAD];
 
-  const int b = blockIdx.z;
+  int bx = blockIdx.x, by = blockIdx.y, b = blockIdx.z;
+#if ORCWA_GEMM_XCD
+  if (gridDim.z >= ORCWA_GEMM_XCD_MIN) {
+    // workgroups go to the 8 XCDs round-robin by linear id: give each XCD
+    // whole matrices so one L2 serves all tiles of a product (large batches
+    // only: a few big products are better spread over every XCD)
+    const unsigned T = gridDim.x * gridDim.y, Bt = gridDim.z;
+    const unsigned w = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned full = (Bt / 8) * 8 * T;
+    unsigned tile;
+    if (w < full) {
+      const unsigned slot = w / 8;
+      tile = slot % T;
+      b = (int)((slot / T) * 8 + (w % 8));
+    } else {
+      const unsigned r = w - full;
+      tile = r % T;
+      b = (int)((Bt / 8) * 8 + r / T);
+    }
+    bx = (int)(tile % gridDim.x);
+    by = (int)(tile / gridDim.x);
+  }
+#endif
   const cd* __restrict__ A = Ap[b] + offA;
   const cd* __restrict__ B = Bp[b] + offB;
   cd* __restrict__ C = Cp[b] + offC;
-  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+  const int m0 = bx * TM, n0 = by * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave / WC) * (16 * BI), wn = (wave % WC) * (16 * BJ);
   const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
@@ -63,20 +101,20 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
 #pragma unroll
   for (int j = 0; j < LA; ++j) {
     const int e = tid + 256 * j;
-    amm[j] = a_mfast ? (e % TM) : (e >> 4);
-    akk[j] = a_mfast ? (e / TM) : (e & 15);
+    amm[j] = a_mfast ? (e % TM) : (e / GEMM_KS);
+    akk[j] = a_mfast ? (e / TM) : (e % GEMM_KS);
     aok[j] = (m0 + amm[j] < M);
     pa[j] = A + (long long)(m0 + amm[j]) * rsA + (long long)akk[j] * csA;
   }
 #pragma unroll
   for (int j = 0; j < LB; ++j) {
     const int e = tid + 256 * j;
-    bkk[j] = b_kfast ? (e & 15) : (e / TN);
-    bnn[j] = b_kfast ? (e >> 4) : (e % TN);
+    bkk[j] = b_kfast ? (e % GEMM_KS) : (e / TN);
+    bnn[j] = b_kfast ? (e / GEMM_KS) : (e % TN);
     bok[j] = (n0 + bnn[j] < N);
     pb[j] = B + (long long)bkk[j] * rsB + (long long)(n0 + bnn[j]) * csB;
   }
-  const long long stepA = 16 * csA, stepB = 16 * rsB;
+  const long long stepA = GEMM_KS * csA, stepB = GEMM_KS * rsB;
   cd ra[LA], rb[LB];
 #pragma unroll
   for (int j = 0; j < LA; ++j) ra[j] = (aok[j] && akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
@@ -95,7 +133,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
       acc3[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  for (int k0 = 0; k0 < K; k0 += GEMM_KS) {
 #pragma unroll
     for (int j = 0; j < LA; ++j) {
       sAr[akk[j]][amm[j]] = ra[j].re;
@@ -108,20 +146,20 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
     }
     __syncthreads();
     // next K step in flight while this one runs on the MFMA units
-    if (k0 + 16 < K) {
+    if (k0 + GEMM_KS < K) {
 #pragma unroll
       for (int j = 0; j < LA; ++j) {
         pa[j] += stepA;
-        ra[j] = (aok[j] && k0 + 16 + akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
+        ra[j] = (aok[j] && k0 + GEMM_KS + akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
       }
 #pragma unroll
       for (int j = 0; j < LB; ++j) {
         pb[j] += stepB;
-        rb[j] = (bok[j] && k0 + 16 + bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
+        rb[j] = (bok[j] && k0 + GEMM_KS + bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
       }
     }
 #pragma unroll
-    for (int kq = 0; kq < 16; kq += 4) {
+    for (int kq = 0; kq < GEMM_KS; kq += 4) {
       const int ka = kq + (lane >> 4);
       double ar[BI], ai[BI], br[BJ], bi[BJ];
 #pragma unroll

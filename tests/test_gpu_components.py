@@ -28,6 +28,20 @@ def test_zgemm_shapes(gpu_ctx, M, N, K):
     assert np.abs(out - ref).max() / scale < 1e-14
 
 
+@pytest.mark.parametrize("M,N,K,batch", [(64, 64, 16, 64), (130, 70, 40, 67), (20, 150, 33, 131),
+                                         (450, 450, 450, 72)])
+def test_zgemm_large_batches(gpu_ctx, M, N, K, batch):
+    # batches of 64 or more products take the XCD-grouped tile order; batch
+    # counts that are not a multiple of 8 leave a remainder in plain order
+    A, B, C = crand(batch, M, K), crand(batch, K, N), crand(batch, M, N)
+    al, be = 0.3 + 0.9j, 0.5 - 0.25j
+    out = g.zgemm(gpu_ctx, A, B, C, al, be)
+    ref = al * A @ B + be * C
+    scale = np.abs(A).max() * np.abs(B).max() * K + np.abs(C).max()
+    err = np.abs(out - ref).max(axis=(1, 2)) / scale
+    assert err.max() < 1e-14, np.argmax(err)
+
+
 @pytest.mark.parametrize("tA,cA,tB,cB", [(1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 1), (1, 1, 1, 0)])
 def test_zgemm_ops(gpu_ctx, tA, cA, tB, cB):
     M, N, K = 37, 29, 41

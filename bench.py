@@ -66,18 +66,19 @@ def cpu_baseline(w, sample_idx, cores):
 def bench_uneven(args):
     """The uneven batch (1024 stacks, N 3..12, 1..10 layers, Lorentz and metal
     layers): one batch per (order count, layer count) group, built, run and
-    released in turn (together the groups' workspaces exceed the device
-    memory), so a step includes each group's plan build and host copies."""
+    released in turn on each of --streams contexts (together the groups'
+    workspaces exceed the device memory), so a step includes each group's plan
+    build and host copies."""
     import numpy as np
     from openrcwa_amd import engine, workloads
     stacks = workloads.uneven_stacks(1024)
     ctx = engine.Context(0)
     print(f"[bench] workload uneven: {len(stacks)} stacks", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        engine.solve_uneven(ctx, stacks)
+        engine.solve_uneven(ctx, stacks, streams=args.streams)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = engine.solve_uneven(ctx, stacks)
+        res = engine.solve_uneven(ctx, stacks, streams=args.streams)
     dt = time.perf_counter() - t0
     status = sum(1 for r in res if r[2] != 0)
     ngroups = len({(s["nX"], len(s["thickness"])) for s in stacks})
@@ -89,8 +90,9 @@ def bench_uneven(args):
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64 (complex128)", "data": "synthetic (seeded splitmix64)",
-           "config": {"workload": f"uneven: {len(stacks)} stacks in {ngroups} batches, plan "
-                                  f"builds and host copies inside the step",
+           "config": {"workload": f"uneven: {len(stacks)} stacks in {ngroups} batches on "
+                                  f"{args.streams} streams, plan builds and host copies inside "
+                                  f"the step",
                       "batch": len(stacks)},
            "status_nonzero": status,
            "alg_tflops_whole_solve": flops / (dt / args.steps) / 1e12}
@@ -106,6 +108,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-freq", type=int, default=0,
                     help="frequencies (default: 256 for pillar2d, 64 for large2d)")
+    ap.add_argument("--streams", type=int, default=8,
+                    help="uneven workload: contexts (streams) the groups are spread over")
     ap.add_argument("--workload", choices=["pillar2d", "large2d", "uneven"], default="pillar2d",
                     help="pillar2d: the metric's 15x15-order batch; large2d: 31x31 orders, "
                          "six patterned layers, 64 frequencies (no CPU baseline: one "

@@ -3,7 +3,33 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+#include <vector>
+
 struct orcwa_gpu_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Device buffers of destroyed plans, kept for later plans on this context:
+  // hipFree waits for the whole device, so releasing a plan's workspace while
+  // other contexts' streams run would stall them all.
+  struct Cached {
+    void* ptr;
+    size_t bytes;
+  };
+  std::vector<Cached> cache;
+  size_t cached_bytes = 0;
 };
+
+namespace orcwa {
+
+// A device buffer of at least `bytes` for a plan on this context: the
+// smallest cached buffer that fits, else a new hipMalloc (after the cache is
+// freed, if the first attempt fails).  *cap receives the buffer's size.
+hipError_t ctx_alloc(orcwa_gpu_context* c, size_t bytes, void** out, size_t* cap);
+// Give a buffer back to the context (it stays allocated until the context is
+// destroyed, or the cache is trimmed).
+void ctx_release(orcwa_gpu_context* c, void* ptr, size_t cap);
+// Free every cached buffer.
+void ctx_trim(orcwa_gpu_context* c);
+
+}  // namespace orcwa

@@ -99,9 +99,11 @@ struct orcwa_plan {
   int chunk = 0, nchunks = 0, pmax = 0, umax = 0;
   std::vector<ChunkTables> chunks;
 
-  // device memory
+  // device memory (from the context's cache)
   char* d_ws = nullptr;            // workspace arena
   char* d_blob = nullptr;          // tables
+  size_t ws_cap = 0, blob_cap = 0;
+  size_t t_ident = 0;              // table {d_ident} for the identity fill
   Blob blob;
   // fixed device regions (pointers inside d_ws)
   double *d_cx = nullptr, *d_cy = nullptr;
@@ -203,6 +205,60 @@ ConvGeom geom_of(const orcwa_plan* p) {
 
 }  // namespace
 
+namespace orcwa {
+
+hipError_t ctx_alloc(orcwa_gpu_context* c, size_t bytes, void** out, size_t* cap) {
+  int best = -1;
+  for (int i = 0; i < (int)c->cache.size(); ++i)
+    if (c->cache[i].bytes >= bytes && (best < 0 || c->cache[i].bytes < c->cache[best].bytes))
+      best = i;
+  if (best >= 0) {
+    *out = c->cache[best].ptr;
+    *cap = c->cache[best].bytes;
+    c->cached_bytes -= *cap;
+    c->cache.erase(c->cache.begin() + best);
+    return hipSuccess;
+  }
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess && !c->cache.empty()) {
+    (void)hipGetLastError();
+    ctx_trim(c);
+    e = hipMalloc(out, bytes);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    *cap = 0;
+    return e;
+  }
+  *cap = bytes;
+  return hipSuccess;
+}
+
+void ctx_release(orcwa_gpu_context* c, void* ptr, size_t cap) {
+  if (!ptr) return;
+  c->cache.push_back({ptr, cap});
+  c->cached_bytes += cap;
+  // keep the four largest
+  constexpr size_t kKeep = 4;
+  while (c->cache.size() > kKeep) {
+    int small = 0;
+    for (int i = 1; i < (int)c->cache.size(); ++i)
+      if (c->cache[i].bytes < c->cache[small].bytes) small = i;
+    (void)hipFree(c->cache[small].ptr);
+    c->cached_bytes -= c->cache[small].bytes;
+    c->cache.erase(c->cache.begin() + small);
+  }
+}
+
+void ctx_trim(orcwa_gpu_context* c) {
+  for (auto& b : c->cache) (void)hipFree(b.ptr);
+  c->cache.clear();
+  c->cached_bytes = 0;
+}
+
+}  // namespace orcwa
+
 extern "C" {
 
 int orcwa_gpu_create(orcwa_gpu_context** ctx, int device) {
@@ -229,6 +285,8 @@ int orcwa_gpu_create(orcwa_gpu_context** ctx, int device) {
 int orcwa_gpu_destroy(orcwa_gpu_context* ctx) {
   if (!ctx) return ORCWA_OK;
   hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  ctx_trim(ctx);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return ORCWA_OK;
@@ -237,8 +295,10 @@ int orcwa_gpu_destroy(orcwa_gpu_context* ctx) {
 int orcwa_plan_destroy(orcwa_plan* p) {
   if (!p) return ORCWA_OK;
   hipSetDevice(p->ctx->device);
-  if (p->d_ws) hipFree(p->d_ws);
-  if (p->d_blob) hipFree(p->d_blob);
+  // the buffers go back to the context once the plan's work is done
+  hipStreamSynchronize(p->ctx->stream);
+  ctx_release(p->ctx, p->d_ws, p->ws_cap);
+  ctx_release(p->ctx, p->d_blob, p->blob_cap);
   if (p->have_events)
     for (auto& e : p->ev) hipEventDestroy(e);
   delete p;
@@ -471,7 +531,10 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   if (const char* s = getenv("ORCWA_WORKSPACE_GB")) budget_gb = atof(s);
   if (!(budget_gb > 0.0)) {
     size_t fr = 0, tot = 0;
-    budget_gb = (hipMemGetInfo(&fr, &tot) == hipSuccess) ? 0.7 * (double)fr / 1e9 : 8.0;
+    // (buffers cached by the context count as free)
+    budget_gb = (hipMemGetInfo(&fr, &tot) == hipSuccess)
+                    ? 0.7 * (double)(fr + ctx->cached_bytes) / 1e9
+                    : 8.0;
   }
   const int islots = L + 3;      // Rt x2, E, H, one LU per cascade step
   const size_t per_inst = (size_t)(7 * lp_max + lu_max + islots) * slot;
@@ -561,7 +624,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   const size_t o_kp = take(p->keep_pq ? (size_t)L * slot : 0);
   const size_t ws_bytes = align256(off);
 
-  if (hipMalloc(&p->d_ws, ws_bytes) != hipSuccess) {
+  if (ctx_alloc(ctx, ws_bytes, (void**)&p->d_ws, &p->ws_cap) != hipSuccess) {
     delete p;
     fprintf(stderr, "[openrcwa] could not allocate %.2f GB of device workspace\n", ws_bytes / 1e9);
     return ORCWA_ERR_NOMEM;
@@ -899,22 +962,15 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
     ct.rt_params = B.put(rp);
     p->chunks.push_back(std::move(ct));
   }
-  if (hipMalloc(&p->d_blob, std::max<size_t>(256, B.host.size())) != hipSuccess) {
+  p->t_ident = B.put(std::vector<cd*>{p->d_ident});
+  if (ctx_alloc(ctx, std::max<size_t>(256, B.host.size()), (void**)&p->d_blob, &p->blob_cap) !=
+      hipSuccess) {
     orcwa_plan_destroy(p);
     return ORCWA_ERR_NOMEM;
   }
   HIPCHK(hipMemcpyAsync(p->d_blob, B.host.data(), B.host.size(), hipMemcpyHostToDevice, s));
   // identity matrix
-  {
-    std::vector<cd*> one{p->d_ident};
-    // small host->device table for the identity fill
-    cd** d_tab = nullptr;
-    HIPCHK(hipMalloc(&d_tab, sizeof(cd*)));
-    HIPCHK(hipMemcpyAsync(d_tab, one.data(), sizeof(cd*), hipMemcpyHostToDevice, s));
-    set_identity_batched(s, n, d_tab, 1);
-    HIPCHK(hipStreamSynchronize(s));
-    hipFree(d_tab);
-  }
+  set_identity_batched(s, n, p->dev<cd* const*>(p->t_ident), 1);
   for (auto& e : p->ev) HIPCHK(hipEventCreate(&e));
   p->have_events = true;
   HIPCHK(hipStreamSynchronize(s));
@@ -1333,6 +1389,7 @@ int orcwa_gpu_field_image(orcwa_gpu_context* ctx, const orcwa_stack* st,
   const size_t o_X = take((size_t)nrh * n * sizeof(cd));
   const size_t o_Hm = take((size_t)nrh * h * sizeof(cd));
   const size_t o_img = take((size_t)nres2 * nres1 * sizeof(cd));
+  ctx_trim(ctx);   // buffers cached by earlier plans go back to the device
   if (hipMalloc(&guard.buf, align256(off)) != hipSuccess) return ORCWA_ERR_NOMEM;
   char* B = (char*)guard.buf;
   cd* cp = (cd*)(B + o_cp);

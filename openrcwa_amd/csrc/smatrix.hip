@@ -386,6 +386,7 @@ int orcwa_gpu_slab_smatrix(orcwa_gpu_context* ctx, int n, int batch, const doubl
     return ORCWA_ERR_ARG;
   }
   SCHK(hipSetDevice(ctx->device));
+  ctx_trim(ctx);   // buffers cached by earlier plans go back to the device
   hipStream_t s = ctx->stream;
   // slots: 0 W/LU, 1 V/LU, 2 F0, 3 G0, 4-6 [invAB | invAXA | invAXB], 7 A/LU,
   //        8-9 [Rud | Tuu], 10 I - invAXB^2 / LU
@@ -432,6 +433,7 @@ int orcwa_gpu_redheffer_star(orcwa_gpu_context* ctx, int n, int batch, const dou
     return ORCWA_ERR_ARG;
   }
   SCHK(hipSetDevice(ctx->device));
+  ctx_trim(ctx);   // buffers cached by earlier plans go back to the device
   hipStream_t s = ctx->stream;
   Slots S;
   int rc = S.init(s, n, 17, batch);
@@ -472,6 +474,7 @@ int orcwa_gpu_rcwa_integrate(orcwa_gpu_context* ctx, const orcwa_zstack* zs,
     return ORCWA_ERR_ARG;
   }
   SCHK(hipSetDevice(ctx->device));
+  ctx_trim(ctx);   // buffers cached by earlier plans go back to the device
   hipStream_t s = ctx->stream;
   const int N = zs->n_slices, L = N + 2;
   LayerModes M;
@@ -568,6 +571,7 @@ int orcwa_gpu_redheffer_integrate(orcwa_gpu_context* ctx, const orcwa_zsampler* 
     return ORCWA_ERR_ARG;
   }
   SCHK(hipSetDevice(ctx->device));
+  ctx_trim(ctx);   // buffers cached by earlier plans go back to the device
   hipStream_t s = ctx->stream;
   const int ncx = zs->n_cx, ncy = zs->n_cy, ncell = (ncx - 1) * (ncy - 1);
   const double k0 = 2.0 * kPi / zs->lambda;

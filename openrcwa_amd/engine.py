@@ -26,6 +26,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self._lib = _lib.load()
+        self.device = int(device)
         h = c_void_p()
         _lib.check(self._lib.orcwa_gpu_create(ctypes.byref(h), int(device)), "orcwa_gpu_create")
         self._h = h
@@ -196,20 +197,20 @@ def solve_batch(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0
         p.close()
 
 
-def uneven_plans(ctx: Context, stacks):
-    """Plans for independent stacks of different order counts, layer counts,
-    cell layouts and thicknesses (dicts as workloads.uneven_stacks makes
-    them): grouped by (Nx, Ny, layers, periods), each group one batch whose
-    instances carry their own edges (padded to the group's edge count with
-    empty cells) and thicknesses.  Yields (plan, indices) one group at a time
-    (a group's workspace is released when its plan is closed)."""
+def _uneven_groups(stacks):
+    """Group independent stacks by (Nx, Ny, layers, periods); per group the
+    host arrays of one batch whose instances carry their own edges (padded to
+    the group's edge count with empty cells) and thicknesses.  Largest
+    groups (by n^3 x layers x stacks) first."""
     groups = {}
     for i, s_ in enumerate(stacks):
         cx, cy = np.asarray(s_["coord_x"]), np.asarray(s_["coord_y"])
         key = (int(s_["nX"]), int(s_["nY"]), len(s_["thickness"]), float(cx[-1] - cx[0]),
                float(cy[-1] - cy[0]))
         groups.setdefault(key, []).append(i)
-    for (nX, nY, L, _, _), idx in groups.items():
+    order = sorted(groups.items(), key=lambda kv: -((2 * kv[0][0] + 1) * (2 * kv[0][1] + 1)) ** 3
+                   * kv[0][2] * len(kv[1]))
+    for (nX, nY, L, _, _), idx in order:
         mx = max(len(stacks[i]["coord_x"]) for i in idx)
         my = max(len(stacks[i]["coord_y"]) for i in idx)
         cxs, cys, eps = [], [], []
@@ -226,18 +227,39 @@ def uneven_plans(ctx: Context, stacks):
             e = np.concatenate([e, np.repeat(e[:, -1:, :], py, axis=1)], axis=1)
             eps.append(e)
         thick = np.array([np.asarray(stacks[i]["thickness"], float) for i in idx])
-        yield (Plan(ctx, nX, nY, cxs[0], cys[0], thick[0],
-                    [stacks[i]["lam"] for i in idx], [stacks[i]["kx0"] for i in idx],
-                    [stacks[i]["ky0"] for i in idx], np.stack(eps),
-                    np.stack([stacks[i]["pol"] for i in idx]),
-                    inst_coords=(np.stack(cxs), np.stack(cys)), inst_thickness=thick), idx)
+        yield dict(nX=nX, nY=nY, coord_x=cxs[0], coord_y=cys[0], thickness=thick[0],
+                   lam=[stacks[i]["lam"] for i in idx], kx0=[stacks[i]["kx0"] for i in idx],
+                   ky0=[stacks[i]["ky0"] for i in idx], eps=np.stack(eps),
+                   pol=np.stack([stacks[i]["pol"] for i in idx]),
+                   inst_coords=(np.stack(cxs), np.stack(cys)), inst_thickness=thick), idx
 
 
-def solve_uneven(ctx: Context, stacks):
-    """uneven_plans + execute + results.  Returns a list of (ref (npol, h),
-    trn (npol, h), status) in the input order."""
+def _group_plan(ctx: Context, g):
+    return Plan(ctx, g["nX"], g["nY"], g["coord_x"], g["coord_y"], g["thickness"], g["lam"],
+                g["kx0"], g["ky0"], g["eps"], g["pol"], inst_coords=g["inst_coords"],
+                inst_thickness=g["inst_thickness"])
+
+
+def uneven_plans(ctx: Context, stacks):
+    """Plans for independent stacks of different order counts, layer counts,
+    cell layouts and thicknesses (dicts as workloads.uneven_stacks makes
+    them), one per (Nx, Ny, layers, periods) group.  Yields (plan, indices)
+    one group at a time (a group's workspace is released when its plan is
+    closed)."""
+    for g, idx in _uneven_groups(stacks):
+        yield _group_plan(ctx, g), idx
+
+
+def solve_uneven(ctx: Context, stacks, streams: int = 1):
+    """Solve independent stacks of different sizes.  Returns a list of (ref
+    (npol, h), trn (npol, h), status) in the input order.  With streams > 1
+    the groups are spread over that many contexts (one stream each, on the
+    device of ctx) driven from as many host threads, so small groups share
+    the GPU instead of running one after another."""
     out = [None] * len(stacks)
-    for p, idx in uneven_plans(ctx, stacks):
+
+    def run(c, g, idx):
+        p = _group_plan(c, g)
         try:
             p.execute()
             p.sync()
@@ -246,6 +268,39 @@ def solve_uneven(ctx: Context, stacks):
             p.close()
         for k, i in enumerate(idx):
             out[i] = (r.ref[k], r.trn[k], int(r.status[k]))
+
+    if streams <= 1:
+        for g, idx in _uneven_groups(stacks):
+            run(ctx, g, idx)
+        return out
+    import queue
+    import threading
+    work = queue.Queue()
+    for item in _uneven_groups(stacks):
+        work.put(item)
+    ctxs = [ctx] + [Context(ctx.device) for _ in range(streams - 1)]
+    errors = []
+
+    def worker(c):
+        while not errors:
+            try:
+                g, idx = work.get_nowait()
+            except queue.Empty:
+                return
+            try:
+                run(c, g, idx)
+            except Exception as e:      # first failure stops every worker
+                errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(c,)) for c in ctxs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for c in ctxs[1:]:
+        c.close()
+    if errors:
+        raise errors[0]
     return out
 
 

@@ -233,3 +233,13 @@ def test_uneven_stacks_against_oracle(gpu_ctx):
         for q in range(2):
             assert np.abs(ref[q] - REF[q]).max() < 1e-9
             assert np.abs(trn[q] - TRN[q]).max() < 1e-9
+    # the same groups spread over three contexts (streams) driven from three
+    # host threads: every group is the same batch, so the results are equal
+    res3 = engine.solve_uneven(gpu_ctx, stacks, streams=3)
+    for (ref, trn, st), (ref3, trn3, st3) in zip(res, res3):
+        assert st3 == st
+        assert np.abs(ref3 - ref).max() < 1e-13 and np.abs(trn3 - trn).max() < 1e-13
+    # and again, after the contexts' cached workspaces were released
+    res4 = engine.solve_uneven(gpu_ctx, stacks, streams=2)
+    for (ref, trn, _), (ref4, trn4, _) in zip(res, res4):
+        assert np.abs(ref4 - ref).max() < 1e-13 and np.abs(trn4 - trn).max() < 1e-13

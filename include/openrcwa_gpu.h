@@ -108,7 +108,12 @@ typedef struct {
   int status;                /* orcwa_status bits */
 } orcwa_result;
 
-/* Context: device selection, stream, device memory pool. */
+/* Context: device selection, one stream, device memory pool.  The device
+ * buffers of destroyed plans stay in the context's pool (its four largest) and
+ * are reused by later plans, so plan_destroy never waits for the whole device
+ * (hipFree does); orcwa_gpu_destroy frees them.  Calls on one context must not
+ * overlap; separate contexts may be driven from separate threads.  Replaces the
+ * per-solve construction of RCWASolver (rcwa/RCWASolver.h:67-80). */
 int orcwa_gpu_create(orcwa_gpu_context **ctx, int device);
 int orcwa_gpu_destroy(orcwa_gpu_context *ctx);
 

@@ -52,6 +52,7 @@ int orcwa_dbg_zgemm(orcwa_gpu_context* ctx, int M, int N, int K, const double* a
                     int cB, const double* beta, double* C, int ldC, int batch) {
   if (!ctx || M <= 0 || N <= 0 || K <= 0 || batch <= 0) return ORCWA_ERR_ARG;
   DCHK(hipSetDevice(ctx->device));
+  orcwa::ctx_trim(ctx);   // cached plan buffers go back to the device
   hipStream_t s = ctx->stream;
   const long long eA = (long long)ldA * (tA ? M : K), eB = (long long)ldB * (tB ? K : N),
                   eC = (long long)ldC * N;
@@ -76,6 +77,7 @@ int orcwa_dbg_zgesv(orcwa_gpu_context* ctx, int n, int nrhs, const double* A, co
                     double* X, int* info, int batch) {
   if (!ctx || n <= 0 || nrhs <= 0 || batch <= 0) return ORCWA_ERR_ARG;
   DCHK(hipSetDevice(ctx->device));
+  orcwa::ctx_trim(ctx);   // cached plan buffers go back to the device
   hipStream_t s = ctx->stream;
   DBatch a, b;
   int rc;
@@ -106,6 +108,7 @@ int orcwa_dbg_zgeev(orcwa_gpu_context* ctx, int n, const double* A, double* lam,
                     double* resid, int* info, int batch) {
   if (!ctx || n <= 0 || batch <= 0) return ORCWA_ERR_ARG;
   DCHK(hipSetDevice(ctx->device));
+  orcwa::ctx_trim(ctx);   // cached plan buffers go back to the device
   hipStream_t s = ctx->stream;
   const long long nn = (long long)n * n;
   DBatch a, ao, w, z, x;
@@ -149,6 +152,7 @@ int orcwa_dbg_convolution(orcwa_gpu_context* ctx, int nx_harm, int ny_harm, int 
   const int nx = 2 * nx_harm + 1, ny = 2 * ny_harm + 1;
   if (nx > 63 || ny > 63) return ORCWA_ERR_ARG;
   DCHK(hipSetDevice(ctx->device));
+  orcwa::ctx_trim(ctx);   // cached plan buffers go back to the device
   hipStream_t s = ctx->stream;
   const int h = nx * ny, ncell = (n_cx - 1) * (n_cy - 1);
   const long long rec0 = (long long)nx * nx + (2 * ny - 1) + (2 * nx - 1);

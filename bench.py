@@ -65,23 +65,26 @@ def cpu_baseline(w, sample_idx, cores):
 
 def bench_uneven(args):
     """The uneven batch (1024 stacks, N 3..12, 1..10 layers, Lorentz and metal
-    layers): one batch per (order count, layer count) group, built, run and
-    released in turn on each of --streams contexts (together the groups'
-    workspaces exceed the device memory), so a step includes each group's plan
-    build and host copies."""
+    layers): one batch per order count (shorter stacks padded with
+    zero-thickness exit-medium layers; per order and layer count with
+    --no-pad-layers), built, run and released in turn on each of --streams
+    contexts (together the groups' workspaces exceed the device memory), so a
+    step includes each group's plan build and host copies."""
     import numpy as np
     from openrcwa_amd import engine, workloads
     stacks = workloads.uneven_stacks(1024)
     ctx = engine.Context(0)
     print(f"[bench] workload uneven: {len(stacks)} stacks", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        engine.solve_uneven(ctx, stacks, streams=args.streams)
+        engine.solve_uneven(ctx, stacks, streams=args.streams, pad_layers=not args.no_pad_layers)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = engine.solve_uneven(ctx, stacks, streams=args.streams)
+        res = engine.solve_uneven(ctx, stacks, streams=args.streams,
+                                  pad_layers=not args.no_pad_layers)
     dt = time.perf_counter() - t0
     status = sum(1 for r in res if r[2] != 0)
-    ngroups = len({(s["nX"], len(s["thickness"])) for s in stacks})
+    ngroups = len({(s["nX"], 0 if not args.no_pad_layers else len(s["thickness"]))
+                   for s in stacks})
     flops = sum(solve_flops((2 * s["nX"] + 1) * (2 * s["nY"] + 1),
                             sum(1 for g in s["eps"][1:-1] if not np.all(g == g.flat[0])),
                             len(s["thickness"])) for s in stacks)
@@ -108,8 +111,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-freq", type=int, default=0,
                     help="frequencies (default: 256 for pillar2d, 64 for large2d)")
-    ap.add_argument("--streams", type=int, default=8,
+    ap.add_argument("--streams", type=int, default=1,
                     help="uneven workload: contexts (streams) the groups are spread over")
+    ap.add_argument("--no-pad-layers", action="store_true",
+                    help="uneven workload: one batch per (order count, layer count) instead of "
+                         "padding shorter stacks with zero-thickness exit-medium layers")
     ap.add_argument("--workload", choices=["pillar2d", "large2d", "uneven"], default="pillar2d",
                     help="pillar2d: the metric's 15x15-order batch; large2d: 31x31 orders, "
                          "six patterned layers, 64 frequencies (no CPU baseline: one "

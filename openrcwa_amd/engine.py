@@ -197,27 +197,35 @@ def solve_batch(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0
         p.close()
 
 
-def _uneven_groups(stacks):
-    """Group independent stacks by (Nx, Ny, layers, periods); per group the
-    host arrays of one batch whose instances carry their own edges (padded to
-    the group's edge count with empty cells) and thicknesses.  Largest
+def _uneven_groups(stacks, pad_layers=True):
+    """Group independent stacks by (Nx, Ny, periods) — and layer count unless
+    pad_layers — into batches whose instances carry their own edges (padded to
+    the group's edge count with empty cells: a repeated edge) and thicknesses.
+    With pad_layers, a stack with fewer layers than its group gets
+    zero-thickness copies of its (homogeneous) exit half-space inserted above
+    it: an interface between equal media has E = H = I and a zero-thickness
+    layer a unit phase, so each such cascade step passes R and Tdd through
+    unchanged, and one batch per order count keeps the GPU full.  Largest
     groups (by n^3 x layers x stacks) first."""
     groups = {}
     for i, s_ in enumerate(stacks):
         cx, cy = np.asarray(s_["coord_x"]), np.asarray(s_["coord_y"])
-        key = (int(s_["nX"]), int(s_["nY"]), len(s_["thickness"]), float(cx[-1] - cx[0]),
-               float(cy[-1] - cy[0]))
+        key = (int(s_["nX"]), int(s_["nY"]), 0 if pad_layers else len(s_["thickness"]),
+               float(cx[-1] - cx[0]), float(cy[-1] - cy[0]))
         groups.setdefault(key, []).append(i)
+    nlay = {k: max(len(stacks[i]["thickness"]) for i in idx) for k, idx in groups.items()}
     order = sorted(groups.items(), key=lambda kv: -((2 * kv[0][0] + 1) * (2 * kv[0][1] + 1)) ** 3
-                   * kv[0][2] * len(kv[1]))
-    for (nX, nY, L, _, _), idx in order:
+                   * nlay[kv[0]] * len(kv[1]))
+    for key, idx in order:
+        nX, nY, L = key[0], key[1], nlay[key]
         mx = max(len(stacks[i]["coord_x"]) for i in idx)
         my = max(len(stacks[i]["coord_y"]) for i in idx)
-        cxs, cys, eps = [], [], []
+        cxs, cys, eps, thick = [], [], [], []
         for i in idx:
             s_ = stacks[i]
             cx, cy = np.asarray(s_["coord_x"], float), np.asarray(s_["coord_y"], float)
             e = np.asarray(s_["eps"], complex)
+            t = np.asarray(s_["thickness"], float)
             # empty cells at the far edges: repeat the last edge and the last
             # column / row of every grid
             px, py = mx - cx.size, my - cy.size
@@ -225,13 +233,34 @@ def _uneven_groups(stacks):
             cys.append(np.concatenate([cy, np.full(py, cy[-1])]))
             e = np.concatenate([e, np.repeat(e[:, :, -1:], px, axis=2)], axis=2)
             e = np.concatenate([e, np.repeat(e[:, -1:, :], py, axis=1)], axis=1)
+            pl = L - t.size
+            if pl > 0:
+                if not np.all(e[-1] == e[-1].flat[0]):
+                    raise ValueError("layer padding needs a homogeneous exit half-space")
+                e = np.concatenate([e[:-1], np.repeat(e[-1:], pl, axis=0), e[-1:]], axis=0)
+                t = np.concatenate([t[:-1], np.zeros(pl), t[-1:]])
             eps.append(e)
-        thick = np.array([np.asarray(stacks[i]["thickness"], float) for i in idx])
+            thick.append(t)
+        thick = np.array(thick)
         yield dict(nX=nX, nY=nY, coord_x=cxs[0], coord_y=cys[0], thickness=thick[0],
                    lam=[stacks[i]["lam"] for i in idx], kx0=[stacks[i]["kx0"] for i in idx],
                    ky0=[stacks[i]["ky0"] for i in idx], eps=np.stack(eps),
                    pol=np.stack([stacks[i]["pol"] for i in idx]),
                    inst_coords=(np.stack(cxs), np.stack(cys)), inst_thickness=thick), idx
+
+
+def _device_free_bytes(device: int) -> int:
+    """Free device memory (hipMemGetInfo), or 0 if it cannot be read."""
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        if hip.hipSetDevice(int(device)) != 0:
+            return 0
+        if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+            return 0
+        return int(free.value)
+    except OSError:
+        return 0
 
 
 def _group_plan(ctx: Context, g):
@@ -240,17 +269,17 @@ def _group_plan(ctx: Context, g):
                 inst_thickness=g["inst_thickness"])
 
 
-def uneven_plans(ctx: Context, stacks):
+def uneven_plans(ctx: Context, stacks, pad_layers=True):
     """Plans for independent stacks of different order counts, layer counts,
     cell layouts and thicknesses (dicts as workloads.uneven_stacks makes
-    them), one per (Nx, Ny, layers, periods) group.  Yields (plan, indices)
-    one group at a time (a group's workspace is released when its plan is
-    closed)."""
-    for g, idx in _uneven_groups(stacks):
+    them), one per (Nx, Ny, periods) group (and layer count unless
+    pad_layers).  Yields (plan, indices) one group at a time (a group's
+    workspace is released when its plan is closed)."""
+    for g, idx in _uneven_groups(stacks, pad_layers):
         yield _group_plan(ctx, g), idx
 
 
-def solve_uneven(ctx: Context, stacks, streams: int = 1):
+def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True):
     """Solve independent stacks of different sizes.  Returns a list of (ref
     (npol, h), trn (npol, h), status) in the input order.  With streams > 1
     the groups are spread over that many contexts (one stream each, on the
@@ -270,16 +299,42 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1):
             out[i] = (r.ref[k], r.trn[k], int(r.status[k]))
 
     if streams <= 1:
-        for g, idx in _uneven_groups(stacks):
+        for g, idx in _uneven_groups(stacks, pad_layers):
             run(ctx, g, idx)
         return out
+    import os
     import queue
     import threading
     work = queue.Queue()
-    for item in _uneven_groups(stacks):
+    for item in _uneven_groups(stacks, pad_layers):
         work.put(item)
     ctxs = [ctx] + [Context(ctx.device) for _ in range(streams - 1)]
     errors = []
+    # each plan sizes its workspace from the device memory free when it is
+    # built; concurrent plans get an equal share of what is free now instead
+    # (unless ORCWA_WORKSPACE_GB is set), and are built one at a time
+    create_lock = threading.Lock()
+    share = None
+    if "ORCWA_WORKSPACE_GB" not in os.environ:
+        free = _device_free_bytes(ctx.device)
+        if free:
+            share = 0.8 * free / streams / 1e9
+    plan_of = _group_plan
+
+    def locked_plan(c, g):
+        with create_lock:
+            return plan_of(c, g)
+
+    def run(c, g, idx):     # noqa: F811 (the threaded variant)
+        p = locked_plan(c, g)
+        try:
+            p.execute()
+            p.sync()
+            r = p.results()
+        finally:
+            p.close()
+        for k, i in enumerate(idx):
+            out[i] = (r.ref[k], r.trn[k], int(r.status[k]))
 
     def worker(c):
         while not errors:
@@ -293,12 +348,18 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1):
                 errors.append(e)
 
     threads = [threading.Thread(target=worker, args=(c,)) for c in ctxs]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    for c in ctxs[1:]:
-        c.close()
+    if share is not None:
+        os.environ["ORCWA_WORKSPACE_GB"] = f"{share:.3f}"
+    try:
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    finally:
+        if share is not None:
+            os.environ.pop("ORCWA_WORKSPACE_GB", None)
+        for c in ctxs[1:]:
+            c.close()
     if errors:
         raise errors[0]
     return out

@@ -221,8 +221,9 @@ def _oracle_stack(s_):
 def test_uneven_stacks_against_oracle(gpu_ctx):
     """The uneven batch: stacks of different order counts, layer counts, cell
     layouts and thicknesses, with Lorentz-dispersive and metal-film layers,
-    grouped into batches whose instances carry their own edges and
-    thicknesses (one group holds six stacks of one shape)."""
+    grouped into batches per order count whose instances carry their own
+    edges and thicknesses (shorter stacks padded with zero-thickness
+    exit-medium layers; one group holds six stacks of one shape)."""
     stacks = (workloads.uneven_stacks(8, n_range=(3, 6), layer_range=(1, 5)) +
               workloads.uneven_stacks(6, seed=7, n_range=(4, 4), layer_range=(3, 3)) +
               workloads.uneven_stacks(1, seed=11, n_range=(10, 10), layer_range=(2, 2)))
@@ -243,3 +244,8 @@ def test_uneven_stacks_against_oracle(gpu_ctx):
     res4 = engine.solve_uneven(gpu_ctx, stacks, streams=2)
     for (ref, trn, _), (ref4, trn4, _) in zip(res, res4):
         assert np.abs(ref4 - ref).max() < 1e-13 and np.abs(trn4 - trn).max() < 1e-13
+    # one batch per (order count, layer count) instead of layer padding
+    res5 = engine.solve_uneven(gpu_ctx, stacks, pad_layers=False)
+    for (ref, trn, _), (ref5, trn5, st5) in zip(res, res5):
+        assert st5 == 0
+        assert np.abs(ref5 - ref).max() < 1e-11 and np.abs(trn5 - trn).max() < 1e-11

@@ -1,9 +1,12 @@
 """Host side of uneven batches (no GPU): how engine._uneven_groups splits
-independent stacks into batches, and that its padding of cell edges (a
-repeated edge = an empty cell) leaves the physics unchanged, checked with the
-oracle on the padded and unpadded stacks (RCWASolver::addLayer takes each
-stack's own edges, rcwa/RCWASolver.cpp:108-137)."""
+independent stacks into batches, and that its padding — of cell edges (a
+repeated edge = an empty cell) and of layers (zero-thickness copies of the
+exit half-space) — leaves the physics unchanged, checked with the oracle on
+the padded and unpadded stacks (RCWASolver::addLayer takes each stack's own
+edges, rcwa/RCWASolver.cpp:108-137; the cascade over the layers,
+rcwa/RCWASolver.cpp:378-492)."""
 import numpy as np
+import pytest
 
 from oracle import rcwa_oracle as o
 from openrcwa_amd import engine, workloads
@@ -14,10 +17,11 @@ def _stacks():
             workloads.uneven_stacks(5, seed=9, n_range=(3, 3), layer_range=(2, 2)))
 
 
-def test_groups_cover_every_stack_once():
+@pytest.mark.parametrize("pad_layers", [False, True])
+def test_groups_cover_every_stack_once(pad_layers):
     stacks = _stacks()
     seen = []
-    for g, idx in engine._uneven_groups(stacks):
+    for g, idx in engine._uneven_groups(stacks, pad_layers):
         seen += list(idx)
         L = len(g["thickness"])
         mx, my = g["inst_coords"][0].shape[1], g["inst_coords"][1].shape[1]
@@ -25,15 +29,21 @@ def test_groups_cover_every_stack_once():
         assert g["inst_thickness"].shape == (len(idx), L)
         for k, i in enumerate(idx):
             s_ = stacks[i]
-            assert (s_["nX"], s_["nY"], len(s_["thickness"])) == (g["nX"], g["nY"], L)
+            Ls = len(s_["thickness"])
+            assert (s_["nX"], s_["nY"]) == (g["nX"], g["nY"])
+            assert Ls == L if not pad_layers else Ls <= L
             cx, cy = g["inst_coords"][0][k], g["inst_coords"][1][k]
             assert np.all(np.diff(cx) >= 0) and np.all(np.diff(cy) >= 0)
             # the stack's own edges first, then repeats of its last edge
             assert np.array_equal(cx[:len(s_["coord_x"])], s_["coord_x"])
             assert np.all(cx[len(s_["coord_x"]):] == s_["coord_x"][-1])
-            assert np.array_equal(g["inst_thickness"][k], s_["thickness"])
+            th = g["inst_thickness"][k]
+            assert np.array_equal(th[:Ls - 1], s_["thickness"][:-1]) and th[-1] == 0.0
+            assert np.all(th[Ls - 1:-1] == 0.0)
             ny0, nx0 = s_["eps"].shape[1:]
-            assert np.array_equal(g["eps"][k][:, :ny0, :nx0], s_["eps"])
+            e = g["eps"][k][:, :ny0, :nx0]
+            assert np.array_equal(e[:Ls - 1], s_["eps"][:-1])
+            assert np.all(e[Ls - 1:] == s_["eps"][-1])
     assert sorted(seen) == list(range(len(stacks)))
 
 
@@ -44,16 +54,18 @@ def test_groups_largest_first():
     assert costs == sorted(costs, reverse=True)
 
 
-def test_padding_with_empty_cells_is_exact():
-    # a group whose stacks have different edge counts: solving a padded
-    # instance gives the unpadded stack's R and T order by order
+def test_padding_with_empty_cells_and_layers_is_exact():
+    # stacks padded with empty cells and with zero-thickness exit-medium
+    # layers: solving a padded instance gives the unpadded stack's R and T
+    # order by order
     stacks = _stacks()
     checked = 0
     for g, idx in engine._uneven_groups(stacks):
         for k, i in enumerate(idx):
             s_ = stacks[i]
             cx, cy = g["inst_coords"][0][k], g["inst_coords"][1][k]
-            if cx.size == len(s_["coord_x"]) and cy.size == len(s_["coord_y"]):
+            if (cx.size == len(s_["coord_x"]) or cy.size == len(s_["coord_y"]) or
+                    len(g["thickness"]) < len(s_["thickness"]) + 2):
                 continue
             pols = [tuple(p) for p in s_["pol"]]
             args = (s_["lam"], s_["theta"], s_["phi"])

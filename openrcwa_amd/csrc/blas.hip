@@ -538,12 +538,12 @@ __global__ __launch_bounds__(256) void k_laswp_panel(int n, int k0, int nb, int 
 }
 
 // perm[r] = the original row that ends in row r after all n interchanges
-// (one thread per matrix, PERM_MB matrices per workgroup, in LDS)
+// (one thread per matrix, up to PERM_MB matrices per workgroup, in LDS)
 constexpr int PERM_MB = 16;
 __global__ __launch_bounds__(PERM_MB) void k_ipiv_to_perm(int n, const int* ipiv, int* perm,
                                                           int batch) {
   extern __shared__ int sp[];
-  const int b = blockIdx.x * PERM_MB + threadIdx.x;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
   int* q = sp + (long long)threadIdx.x * n;
   if (b < batch) {
     const int* pv = ipiv + (long long)b * n;
@@ -561,16 +561,16 @@ __global__ __launch_bounds__(PERM_MB) void k_ipiv_to_perm(int n, const int* ipiv
   }
 }
 
-// B <- P^T B as a gather through LDS: 8 columns per workgroup
+// B <- P^T B as a gather through LDS: ncw (<= PERM_NC) columns per workgroup
 constexpr int PERM_NC = 8;
-__global__ __launch_bounds__(256) void k_apply_perm(int n, int nrhs, const int* perm,
+__global__ __launch_bounds__(256) void k_apply_perm(int n, int nrhs, int ncw, const int* perm,
                                                     cd* const* Bp) {
-  extern __shared__ cd X[];   // [PERM_NC][n]
+  extern __shared__ cd X[];   // [ncw][n]
   const int b = blockIdx.y;
-  const int c0 = blockIdx.x * PERM_NC;
+  const int c0 = blockIdx.x * ncw;
   cd* __restrict__ B = Bp[b];
   const int* pm = perm + (long long)b * n;
-  const int nc = min(PERM_NC, nrhs - c0);
+  const int nc = min(ncw, nrhs - c0);
   for (int e = threadIdx.x; e < nc * n; e += 256) {
     const int c = e / n, r = e % n;
     X[c * n + r] = B[r + (long long)(c0 + c) * n];
@@ -670,14 +670,17 @@ void zgetrs_batched(hipStream_t s, int n, int nrhs, const cd* const* LU, const i
   const cd* const* Bc = (const cd* const*)B;
   const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0);
   // all row interchanges: composed once per matrix and applied as a gather
-  // (when a workspace of batch x n ints is given), else swap by swap
-  const size_t perm_lds = (size_t)PERM_MB * n * sizeof(int);
-  const size_t gather_lds = (size_t)PERM_NC * n * sizeof(cd);
-  if (perm_ws && perm_lds <= 64 * 1024 && gather_lds <= 64 * 1024) {
-    hipLaunchKernelGGL(k_ipiv_to_perm, dim3((batch + PERM_MB - 1) / PERM_MB), dim3(PERM_MB),
-                       perm_lds, s, n, ipiv, perm_ws, batch);
-    hipLaunchKernelGGL(k_apply_perm, dim3((nrhs + PERM_NC - 1) / PERM_NC, batch), dim3(256),
-                       gather_lds, s, n, nrhs, (const int*)perm_ws, B);
+  // (when a workspace of batch x n ints is given), else swap by swap; the
+  // matrices per workgroup (composition) and columns per workgroup (gather)
+  // shrink with n so that each stays within 64 KB of LDS
+  constexpr size_t kPermLds = 64 * 1024;
+  const int mb = (int)std::min<size_t>(PERM_MB, kPermLds / ((size_t)n * sizeof(int)));
+  const int ncw = (int)std::min<size_t>(PERM_NC, kPermLds / ((size_t)n * sizeof(cd)));
+  if (perm_ws && mb >= 1 && ncw >= 1) {
+    hipLaunchKernelGGL(k_ipiv_to_perm, dim3((batch + mb - 1) / mb), dim3(mb),
+                       (size_t)mb * n * sizeof(int), s, n, ipiv, perm_ws, batch);
+    hipLaunchKernelGGL(k_apply_perm, dim3((nrhs + ncw - 1) / ncw, batch), dim3(256),
+                       (size_t)ncw * n * sizeof(cd), s, n, nrhs, ncw, (const int*)perm_ws, B);
   } else {
     hipLaunchKernelGGL(k_laswp, dim3((nrhs + 127) / 128, batch), dim3(128), 0, s, n, nrhs, 0, n, 0,
                        nrhs, B, ipiv);

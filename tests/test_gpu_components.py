@@ -59,16 +59,20 @@ def test_zgemm_ops(gpu_ctx, tA, cA, tB, cB):
 @pytest.mark.parametrize("n,nrhs,batch", [(1, 1, 2), (5, 3, 2), (31, 31, 2), (32, 32, 2),
                                           (33, 7, 2), (100, 100, 2), (450, 450, 2),
                                           (97, 40, 3), (450, 900, 3), (513, 9, 2),
-                                          (600, 20, 3)])
+                                          (600, 20, 3), (1300, 11, 2), (2100, 5, 2)])
 def test_zgesv(gpu_ctx, n, nrhs, batch):
     # even batches apply the row interchanges of the solve as one gather,
     # odd ones swap by swap (both paths of zgetrs_batched); panels of more
-    # than 512 rows factor in global memory, shorter ones in registers
+    # than 512 rows factor in global memory, shorter ones in registers; the
+    # gather takes fewer columns per workgroup as n grows (7 at n = 513, 3 at
+    # 1300, 1 at 2100)
     A, B = crand(batch, n, n), crand(batch, n, nrhs)
     X, info = g.zgesv(gpu_ctx, A, B)
     assert not info.any()
     ref = np.linalg.solve(A, B)
-    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-11
+    # (the forward error grows with the condition number, ~n for these
+    # random matrices; the backward error below does not)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-11 * max(1.0, n / 600)
     assert np.abs(A @ X - B).max() / (np.abs(A).max() * np.abs(X).max() * n) < 1e-14
 
 

@@ -280,15 +280,24 @@ def uneven_plans(ctx: Context, stacks, pad_layers=True):
 
 
 def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True):
-    """Solve independent stacks of different sizes.  Returns a list of (ref
-    (npol, h), trn (npol, h), status) in the input order.  With streams > 1
-    the groups are spread over that many contexts (one stream each, on the
-    device of ctx) driven from as many host threads, so small groups share
-    the GPU instead of running one after another."""
+    """Solve independent stacks of different sizes (grouped as
+    _uneven_groups does).  Returns a list of (ref (npol, h), trn (npol, h),
+    status) in the input order.  With streams > 1 the groups are spread over
+    that many contexts (one stream each, on the device of ctx) driven from as
+    many host threads, so small groups share the GPU instead of running one
+    after another; their plans are then built one at a time, each sized to an
+    equal share of the device memory free at the start (unless
+    ORCWA_WORKSPACE_GB is set)."""
+    import os
+    import queue
+    import threading
+
     out = [None] * len(stacks)
+    create_lock = threading.Lock()
 
     def run(c, g, idx):
-        p = _group_plan(c, g)
+        with create_lock:
+            p = _group_plan(c, g)
         try:
             p.execute()
             p.sync()
@@ -302,39 +311,17 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True
         for g, idx in _uneven_groups(stacks, pad_layers):
             run(ctx, g, idx)
         return out
-    import os
-    import queue
-    import threading
+
     work = queue.Queue()
     for item in _uneven_groups(stacks, pad_layers):
         work.put(item)
-    ctxs = [ctx] + [Context(ctx.device) for _ in range(streams - 1)]
-    errors = []
-    # each plan sizes its workspace from the device memory free when it is
-    # built; concurrent plans get an equal share of what is free now instead
-    # (unless ORCWA_WORKSPACE_GB is set), and are built one at a time
-    create_lock = threading.Lock()
     share = None
     if "ORCWA_WORKSPACE_GB" not in os.environ:
         free = _device_free_bytes(ctx.device)
         if free:
             share = 0.8 * free / streams / 1e9
-    plan_of = _group_plan
-
-    def locked_plan(c, g):
-        with create_lock:
-            return plan_of(c, g)
-
-    def run(c, g, idx):     # noqa: F811 (the threaded variant)
-        p = locked_plan(c, g)
-        try:
-            p.execute()
-            p.sync()
-            r = p.results()
-        finally:
-            p.close()
-        for k, i in enumerate(idx):
-            out[i] = (r.ref[k], r.trn[k], int(r.status[k]))
+    ctxs = [ctx] + [Context(ctx.device) for _ in range(streams - 1)]
+    errors = []
 
     def worker(c):
         while not errors:

@@ -292,6 +292,7 @@ __global__ __launch_bounds__(256) void k_getf2_panel(int n, int k0, int nb, cd* 
   __shared__ int s_idx[256];
   __shared__ cd s_inv;
   __shared__ int s_piv;
+  __shared__ cd s_u[NB];
   const int b = blockIdx.x;
   cd* __restrict__ A = Ap[b];
   const int tid = threadIdx.x;
@@ -341,16 +342,17 @@ __global__ __launch_bounds__(256) void k_getf2_panel(int n, int k0, int nb, cd* 
       }
     }
     __syncthreads();
-    const cd inv = s_inv;
-    for (int i = c + 1 + tid; i < n; i += 256) A[i + (long long)c * n] = A[i + (long long)c * n] * inv;
-    __syncthreads();
-    // rank-1 update of the rest of the panel
+    // the pivot row's entries right of column c, staged for every thread
     const int ncol = nb - j - 1;
-    const int nrow = n - c - 1;
-    for (int e = tid; e < ncol * nrow; e += 256) {
-      const int jj = c + 1 + e / nrow;
-      const int i = c + 1 + e % nrow;
-      A[i + (long long)jj * n] -= A[i + (long long)c * n] * A[c + (long long)jj * n];
+    for (int jj = tid; jj < ncol; jj += 256) s_u[jj] = A[c + (long long)(c + 1 + jj) * n];
+    __syncthreads();
+    // scale column c below the pivot and apply the rank-1 update, one row per
+    // thread (its multiplier read once)
+    const cd inv = s_inv;
+    for (int i = c + 1 + tid; i < n; i += 256) {
+      const cd l = A[i + (long long)c * n] * inv;
+      A[i + (long long)c * n] = l;
+      for (int jj = 0; jj < ncol; ++jj) A[i + (long long)(c + 1 + jj) * n] -= l * s_u[jj];
     }
     __syncthreads();
   }

@@ -582,6 +582,134 @@ __device__ void lu_t_solve2(int n, const cd* __restrict__ LU, const int* __restr
   wave_fence();
 }
 
+// The same solve by the whole 256-thread workgroup, in blocks of TS rows:
+// per block the four waves first subtract the contributions of the entries
+// already solved (coalesced column reads of LU, one wave per row), then wave
+// 0 finishes the TS x TS triangle from LDS with one lane per row, each solved
+// entry broadcast by a lane shuffle.  2 n / TS barriers instead of a global
+// load latency per row.  Every thread of the workgroup must call it.
+constexpr int TS = 32;
+__device__ void lu_t_solve2_blocked(int n, const cd* __restrict__ LU,
+                                    const int* __restrict__ piv, cd* x0, cd* x1,
+                                    cd (*blk)[TS + 1]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + TS - 1) / TS;
+  // U^T y = x, forward: y_i = (x_i - sum_{j<i} U(j,i) y_j) / U(i,i)
+  for (int bk = 0; bk < nblk; ++bk) {
+    const int r0 = bk * TS, nb = min(TS, n - r0);
+    for (int e = tid; e < nb * nb; e += 256) {
+      const int a = e % nb, c = e / nb;
+      blk[a][c] = LU[(r0 + a) + (long long)(r0 + c) * n];
+    }
+    for (int r = r0 + wave; r < r0 + nb; r += 4) {
+      const cd* col = LU + (long long)r * n;
+      double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+      for (int j = lane; j < r0; j += 64) {
+        const cd u = col[j], y0 = x0[j], y1 = x1[j];
+        a0r += u.re * y0.re - u.im * y0.im;
+        a0i += u.re * y0.im + u.im * y0.re;
+        a1r += u.re * y1.re - u.im * y1.im;
+        a1i += u.re * y1.im + u.im * y1.re;
+      }
+      a0r = wave_sum(a0r);
+      a0i = wave_sum(a0i);
+      a1r = wave_sum(a1r);
+      a1i = wave_sum(a1i);
+      if (lane == 0) {
+        x0[r] -= mk(a0r, a0i);
+        x1[r] -= mk(a1r, a1i);
+      }
+    }
+    __syncthreads();
+    if (wave == 0) {
+      cd v0 = mk(0.0, 0.0), v1 = mk(0.0, 0.0);
+      if (lane < nb) {
+        v0 = x0[r0 + lane];
+        v1 = x1[r0 + lane];
+      }
+      for (int k = 0; k < nb; ++k) {
+        if (lane == k) {
+          const cd d = cinv(blk[k][k]);
+          v0 = v0 * d;
+          v1 = v1 * d;
+        }
+        const cd y0 = mk(__shfl(v0.re, k, 64), __shfl(v0.im, k, 64));
+        const cd y1 = mk(__shfl(v1.re, k, 64), __shfl(v1.im, k, 64));
+        if (lane > k && lane < nb) {
+          const cd u = blk[k][lane];   // U(r0 + k, r0 + lane)
+          v0 -= u * y0;
+          v1 -= u * y1;
+        }
+      }
+      if (lane < nb) {
+        x0[r0 + lane] = v0;
+        x1[r0 + lane] = v1;
+      }
+    }
+    __syncthreads();
+  }
+  // L^T z = y, backward, unit diagonal: z_i = y_i - sum_{j>i} L(j,i) z_j
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int r0 = bk * TS, nb = min(TS, n - r0), r1 = r0 + nb;
+    for (int e = tid; e < nb * nb; e += 256) {
+      const int a = e % nb, c = e / nb;
+      blk[a][c] = LU[(r0 + a) + (long long)(r0 + c) * n];
+    }
+    for (int r = r0 + wave; r < r1; r += 4) {
+      const cd* col = LU + (long long)r * n;
+      double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+      for (int j = r1 + lane; j < n; j += 64) {
+        const cd l = col[j], z0 = x0[j], z1 = x1[j];
+        a0r += l.re * z0.re - l.im * z0.im;
+        a0i += l.re * z0.im + l.im * z0.re;
+        a1r += l.re * z1.re - l.im * z1.im;
+        a1i += l.re * z1.im + l.im * z1.re;
+      }
+      a0r = wave_sum(a0r);
+      a0i = wave_sum(a0i);
+      a1r = wave_sum(a1r);
+      a1i = wave_sum(a1i);
+      if (lane == 0) {
+        x0[r] -= mk(a0r, a0i);
+        x1[r] -= mk(a1r, a1i);
+      }
+    }
+    __syncthreads();
+    if (wave == 0) {
+      cd v0 = mk(0.0, 0.0), v1 = mk(0.0, 0.0);
+      if (lane < nb) {
+        v0 = x0[r0 + lane];
+        v1 = x1[r0 + lane];
+      }
+      for (int k = nb - 1; k >= 0; --k) {
+        const cd z0 = mk(__shfl(v0.re, k, 64), __shfl(v0.im, k, 64));
+        const cd z1 = mk(__shfl(v1.re, k, 64), __shfl(v1.im, k, 64));
+        if (lane < k) {
+          const cd l = blk[k][lane];   // L(r0 + k, r0 + lane)
+          v0 -= l * z0;
+          v1 -= l * z1;
+        }
+      }
+      if (lane < nb) {
+        x0[r0 + lane] = v0;
+        x1[r0 + lane] = v1;
+      }
+    }
+    __syncthreads();
+  }
+  // x = P z: the interchanges of zgetrf in reverse order
+  if (tid == 0) {
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = piv[k];
+      if (p != k) {
+        cd t = x0[k]; x0[k] = x0[p]; x0[p] = t;
+        t = x1[k]; x1[k] = x1[p]; x1[p] = t;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // Last cascade step (the interface at the incident half-space): for
 // c in {i0, h+i0}: x = (F+G)^-1 e_c, Rud e_c = (F-G) x = B x + A (R~ x),
 // 2 x starts the Tdd column chain.  Rt = R~^T of the previous steps, or null
@@ -651,6 +779,7 @@ __global__ __launch_bounds__(256) void k_cascade_chain(int n, const cd* const* L
                                                        const int* piv_all, const cd* const* keff,
                                                        const double* k0d, cd* rt_all) {
   extern __shared__ cd sm[];
+  __shared__ cd blk[TS][TS + 1];
   cd* x0 = sm;
   cd* x1 = sm + n;
   const int b = blockIdx.x, tid = threadIdx.x;
@@ -664,8 +793,7 @@ __global__ __launch_bounds__(256) void k_cascade_chain(int n, const cd* const* L
     x1[i] = t[n + i] * ph;
   }
   __syncthreads();
-  if ((tid >> 6) == 0) lu_t_solve2(n, LUt[b], piv_all + (long long)b * n, x0, x1);
-  __syncthreads();
+  lu_t_solve2_blocked(n, LUt[b], piv_all + (long long)b * n, x0, x1, blk);
   for (int i = tid; i < n; i += 256) {
     t[i] = 2.0 * x0[i];
     t[n + i] = 2.0 * x1[i];

@@ -16,6 +16,8 @@
 // Units: every wave-vector is divided by k0 = 2 pi / lambda, so P*Q is
 // dimensionless; eigenvectors, V, interface matrices and R/T are unchanged
 // by that scaling, and the phase uses keff * (k0 d).
+#include <algorithm>
+
 #include "rcwa_kernels.h"
 
 namespace orcwa {
@@ -522,76 +524,17 @@ __global__ void k_pv_out(const ConvGeom g, const ProbParam* params, const cd* co
 }
 
 // x <- M^-1 x for the two vectors x0, x1 (LDS), given zgetrf of M^T:
-// M^T = P L U  =>  M = U^T L^T P^T  =>  x = P L^-T U^-T x.  Wave 0 only.
-__device__ void lu_t_solve2(int n, const cd* __restrict__ LU, const int* __restrict__ piv,
-                            cd* x0, cd* x1) {
-  const int lane = threadIdx.x & 63;
-  // U^T y = x, forward: y_i = (x_i - sum_{j<i} U(j,i) y_j) / U(i,i); U(j,i) = column i
-  for (int i = 0; i < n; ++i) {
-    const cd* col = LU + (long long)i * n;
-    double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
-    for (int j = lane; j < i; j += 64) {
-      const cd u = col[j], y0 = x0[j], y1 = x1[j];
-      a0r += u.re * y0.re - u.im * y0.im;
-      a0i += u.re * y0.im + u.im * y0.re;
-      a1r += u.re * y1.re - u.im * y1.im;
-      a1i += u.re * y1.im + u.im * y1.re;
-    }
-    a0r = wave_sum(a0r);
-    a0i = wave_sum(a0i);
-    a1r = wave_sum(a1r);
-    a1i = wave_sum(a1i);
-    if (lane == 0) {
-      const cd d = cinv(col[i]);
-      x0[i] = (x0[i] - mk(a0r, a0i)) * d;
-      x1[i] = (x1[i] - mk(a1r, a1i)) * d;
-    }
-    wave_fence();
-  }
-  // L^T z = y, backward, unit diagonal: z_i = y_i - sum_{j>i} L(j,i) z_j
-  for (int i = n - 2; i >= 0; --i) {
-    const cd* col = LU + (long long)i * n;
-    double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
-    for (int j = i + 1 + lane; j < n; j += 64) {
-      const cd l = col[j], z0 = x0[j], z1 = x1[j];
-      a0r += l.re * z0.re - l.im * z0.im;
-      a0i += l.re * z0.im + l.im * z0.re;
-      a1r += l.re * z1.re - l.im * z1.im;
-      a1i += l.re * z1.im + l.im * z1.re;
-    }
-    a0r = wave_sum(a0r);
-    a0i = wave_sum(a0i);
-    a1r = wave_sum(a1r);
-    a1i = wave_sum(a1i);
-    if (lane == 0) {
-      x0[i] -= mk(a0r, a0i);
-      x1[i] -= mk(a1r, a1i);
-    }
-    wave_fence();
-  }
-  // x = P z: the interchanges of zgetrf in reverse order
-  if (lane == 0) {
-    for (int k = n - 1; k >= 0; --k) {
-      const int p = piv[k];
-      if (p != k) {
-        cd t = x0[k]; x0[k] = x0[p]; x0[p] = t;
-        t = x1[k]; x1[k] = x1[p]; x1[p] = t;
-      }
-    }
-  }
-  wave_fence();
-}
-
-// The same solve by the whole 256-thread workgroup, in blocks of TS rows:
+// M^T = P L U  =>  M = U^T L^T P^T  =>  x = P L^-T U^-T x.
+// Solved by the whole 256-thread workgroup, in blocks of TS rows:
 // per block the four waves first subtract the contributions of the entries
 // already solved (coalesced column reads of LU, one wave per row), then wave
 // 0 finishes the TS x TS triangle from LDS with one lane per row, each solved
 // entry broadcast by a lane shuffle.  2 n / TS barriers instead of a global
 // load latency per row.  Every thread of the workgroup must call it.
 constexpr int TS = 32;
-__device__ void lu_t_solve2_blocked(int n, const cd* __restrict__ LU,
-                                    const int* __restrict__ piv, cd* x0, cd* x1,
-                                    cd (*blk)[TS + 1]) {
+__device__ void lu_t_solve2(int n, const cd* __restrict__ LU,
+                            const int* __restrict__ piv, cd* x0, cd* x1,
+                            cd (*blk)[TS + 1]) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nblk = (n + TS - 1) / TS;
   // U^T y = x, forward: y_i = (x_i - sum_{j<i} U(j,i) y_j) / U(i,i)
@@ -730,8 +673,9 @@ __global__ __launch_bounds__(256) void k_cascade_tail(int n, int i0, const cd* c
     x1[i] = mk(i == h + i0 ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  if (wave == 0) lu_t_solve2(n, LUt[b], piv_all + (long long)b * n, x0, x1);
-  __syncthreads();
+  // (the triangle blocks of the solve live where u0, u1 go afterwards)
+  lu_t_solve2(n, LUt[b], piv_all + (long long)b * n, x0, x1,
+              reinterpret_cast<cd(*)[TS + 1]>(u0));
   // u = R~ x, R~(i, j) = Rt(j, i): column i of Rt
   const cd* Rt = Rtp ? Rtp[b] : nullptr;
   for (int i = wave; i < n; i += 4) {
@@ -793,7 +737,7 @@ __global__ __launch_bounds__(256) void k_cascade_chain(int n, const cd* const* L
     x1[i] = t[n + i] * ph;
   }
   __syncthreads();
-  lu_t_solve2_blocked(n, LUt[b], piv_all + (long long)b * n, x0, x1, blk);
+  lu_t_solve2(n, LUt[b], piv_all + (long long)b * n, x0, x1, blk);
   for (int i = tid; i < n; i += 256) {
     t[i] = 2.0 * x0[i];
     t[n + i] = 2.0 * x1[i];
@@ -1020,8 +964,10 @@ void launch_cascade_tail(hipStream_t s, int n, int i0, const cd* const* LUt, con
                          const cd* const* A, const cd* const* B, const cd* const* Rt, cd* rt,
                          int batch) {
   if (batch <= 0) return;
-  hipLaunchKernelGGL(k_cascade_tail, dim3(batch), dim3(256), 4 * n * sizeof(cd), s, n, i0, LUt,
-                     piv, A, B, Rt, rt);
+  // x0, x1, u0, u1; u0 onwards also holds the TS x (TS + 1) blocks of the solve
+  const size_t lds = (size_t)std::max(4 * n, 2 * n + TS * (TS + 1)) * sizeof(cd);
+  hipLaunchKernelGGL(k_cascade_tail, dim3(batch), dim3(256), lds, s, n, i0, LUt, piv, A, B, Rt,
+                     rt);
 }
 
 void launch_cascade_chain(hipStream_t s, int n, const cd* const* LUt, const int* piv,

@@ -38,29 +38,43 @@ def solve_flops(h, n_pat, n_layers):
         (n_layers - 1) * (2 * (8.0 / 3.0 + 8.0) + 40.0) * n ** 3
 
 
+def _cpu_init():
+    # one BLAS thread per worker process: the oracle's numpy/LAPACK calls then
+    # run the reference's single-threaded sequence (sol/rcwa_bridge.cpp:244)
+    from threadpoolctl import threadpool_limits
+    global _CPU_LIMIT
+    _CPU_LIMIT = threadpool_limits(1)
+
+
 def _cpu_worker(args):
     w, i = args
-    from threadpoolctl import threadpool_limits
     from oracle import rcwa_oracle as o
-    with threadpool_limits(1):
-        th, ph = w["angles"][i]
-        t0 = time.perf_counter()
-        REF, TRN, _ = o.solve_stack(w["nX"], w["nY"], w["period"], w["period"], w["coord_x"],
-                                    w["coord_y"], list(w["eps"]), w["thickness"], w["lam"][i],
-                                    th, ph)
-        return i, REF, TRN, time.perf_counter() - t0
+    th, ph = w["angles"][i]
+    t0 = time.perf_counter()
+    REF, TRN, _ = o.solve_stack(w["nX"], w["nY"], w["period"], w["period"], w["coord_x"],
+                                w["coord_y"], list(w["eps"]), w["thickness"], w["lam"][i], th, ph)
+    return i, REF, TRN, time.perf_counter() - t0
 
 
-def cpu_baseline(w, sample_idx, cores):
+def cpu_baseline(w, sample_idx, cores, runs):
     """FP64 oracle (numpy/LAPACK zgeev, the reference's operation sequence) on
-    a bounded sample of the same workload, one BLAS thread per worker process."""
+    a bounded sample of the same workload, one BLAS thread per worker process.
+    The pool is created and every worker runs one warm-up solve before the
+    timed region; the sample is then timed `runs` times and the median wall
+    time is the baseline (BASELINE.md protocol: warm-up, median of >= 5)."""
     import multiprocessing as mp
     ctx = mp.get_context("spawn")
-    t0 = time.perf_counter()
-    with ctx.Pool(cores) as pool:
-        out = pool.map(_cpu_worker, [(w, int(i)) for i in sample_idx])
-    wall = time.perf_counter() - t0
-    return out, wall
+    with ctx.Pool(cores, initializer=_cpu_init) as pool:
+        pool.map(_cpu_worker, [(w, int(sample_idx[k % len(sample_idx)])) for k in range(cores)],
+                 chunksize=1)
+        walls, per = [], []
+        out = None
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            out = pool.map(_cpu_worker, [(w, int(i)) for i in sample_idx], chunksize=1)
+            walls.append(time.perf_counter() - t0)
+            per.extend(o_[3] for o_ in out)
+    return out, walls, per
 
 
 def bench_uneven(args):
@@ -107,7 +121,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=0, help="instances timed on the CPU")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="instances timed on the CPU (default 64)")
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="timed runs of the CPU sample; the median is reported")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-freq", type=int, default=0,
                     help="frequencies (default: 256 for pillar2d, 64 for large2d)")
@@ -207,26 +224,34 @@ def main():
         print(f"[bench] host-to-host one-shot: {dtp:.2f} s", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
+        import statistics
         cores = min(16, os.cpu_count() or 1)
-        S = args.cpu_sample or 2 * cores
+        S = args.cpu_sample or 64
         rng = np.random.default_rng(0)
         sample = np.sort(rng.choice(B, size=min(S, B), replace=False))
-        print(f"[bench] cpu baseline: {len(sample)} solves on {cores} processes", file=sys.stderr,
-              flush=True)
-        outs, wall = cpu_baseline(w, sample, cores)
+        print(f"[bench] cpu baseline: {len(sample)} solves x {args.cpu_runs} runs on {cores} "
+              f"processes", file=sys.stderr, flush=True)
+        outs, walls, per = cpu_baseline(w, sample, cores, args.cpu_runs)
         err = 0.0
         for i, REF, TRN, _ in outs:
             for p in range(2):
                 err = max(err, np.abs(res.ref[i, p] - REF[p]).max(),
                           np.abs(res.trn[i, p] - TRN[p]).max())
-        per = float(np.median([o_[3] for o_ in outs]))
+        wall = statistics.median(walls)
+        per_solve = statistics.median(per)
+        cpu_rate = len(sample) / wall
         out["max_abs_dRdT_vs_fp64_oracle"] = err
-        out["cpu_baseline"] = {"value": len(sample) / wall, "unit": "solves/s", "cores": cores,
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "solves/s", "cores": cores,
                                "kind": "port",
                                "sample": f"{len(sample)} random instances of the same 2048-solve "
-                                         f"batch; numpy/LAPACK zgeev oracle, 1 BLAS thread per "
-                                         f"process; median {per:.2f} s per solve per core"}
-        out["gpu_over_cpu"] = value / (len(sample) / wall)
+                                         f"batch, numpy/LAPACK zgeev oracle with 1 BLAS thread "
+                                         f"in each of {cores} processes (pool warmed before "
+                                         f"timing); median wall of {len(walls)} runs "
+                                         f"{wall:.2f} s (all: "
+                                         f"{', '.join(f'{x:.2f}' for x in walls)})",
+                               "single_thread_s_per_solve": per_solve,
+                               "single_thread_solves_per_s": 1.0 / per_solve}
+        out["gpu_over_cpu"] = value / cpu_rate
     print(json.dumps(out), flush=True)
 
 

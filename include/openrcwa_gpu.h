@@ -25,6 +25,7 @@
 #ifndef OPENRCWA_GPU_H
 #define OPENRCWA_GPU_H
 
+#include <stddef.h>
 #include <stdio.h>
 
 #ifdef __cplusplus
@@ -50,7 +51,18 @@ enum orcwa_status {
                                      (rcwa/RCWASolver.cpp:1843-1862, 1898-1901);
                                      ref/trn are left zero */
   ORCWA_STATUS_SINGULAR = 4,      /* exactly singular pivot in an interface/cascade solve */
-  ORCWA_STATUS_NOCONV = 8         /* the QR iteration hit its iteration limit */
+  ORCWA_STATUS_NOCONV = 8,        /* the QR iteration hit its iteration limit */
+  ORCWA_STATUS_EIG_PERTURBED = 16,/* a layer's eigen solve failed the residual test (or did
+                                     not converge) and was redone, as the reference does, on
+                                     A + diag(1e-11 (i+1) max|A|) (rcwa/MKLEigenSolver.hpp:
+                                     81-93, 160-176); EIG_RESIDUAL stays set only if the
+                                     redone solve also fails the test against A */
+  ORCWA_STATUS_ILLCOND = 32       /* a layer carries a mode that grows by more than 1e4 across
+                                     its thickness (Im keff < 0 under the GammaSqrtOp branch of
+                                     rcwa/WaveEquationSolver.cpp:13-21): the reference's RTCM
+                                     cascade (rcwa/RCWASolver.cpp:415-460) then amplifies
+                                     rounding by about the square of that growth, and the
+                                     result may differ from any other FP64 evaluation */
 };
 
 typedef struct orcwa_gpu_context orcwa_gpu_context;
@@ -115,7 +127,17 @@ typedef struct {
  * overlap; separate contexts may be driven from separate threads.  Replaces the
  * per-solve construction of RCWASolver (rcwa/RCWASolver.h:67-80). */
 int orcwa_gpu_create(orcwa_gpu_context **ctx, int device);
+/* Destroy a context.  Every plan created on it must be destroyed first: with
+ * live plans the call returns ORCWA_ERR_ARG and destroys nothing. */
 int orcwa_gpu_destroy(orcwa_gpu_context *ctx);
+
+/* Device-memory budget (bytes) of the plans later created on this context: a
+ * plan splits its batch into chunks whose workspace fits it.  0 (the default)
+ * means the ORCWA_WORKSPACE_GB environment value if set, else 70 % of the
+ * device memory free at plan creation.  Lets several contexts driven from
+ * separate threads share one device.  No reference counterpart (the
+ * reference solves one instance at a time). */
+int orcwa_gpu_set_workspace_limit(orcwa_gpu_context *ctx, size_t bytes);
 
 /* One-shot batched solve: plan + execute + results + destroy.
  * Replaces the per-wavelength loop of sol/rcwa_bridge.cpp:244-334 and

@@ -34,6 +34,13 @@ int orcwa_dbg_zgesv(orcwa_gpu_context *ctx, int n, int nrhs, const double *A, co
 int orcwa_dbg_zgeev(orcwa_gpu_context *ctx, int n, const double *A, double *lam, double *W,
                     double *resid, int *info, int batch);
 
+/* Residual acceptance of the eigen solves of later plans and zgeev calls on
+ * this context (default 1e-8, rcwa/MKLEigenSolver.hpp:75-79).  A negative
+ * tolerance rejects every solve: it forces the EIG_RESIDUAL status and, in
+ * plans, the perturbed recovery solve (ORCWA_STATUS_EIG_PERTURBED), so tests
+ * can exercise those paths on ordinary inputs. */
+int orcwa_dbg_set_eig_tolerance(orcwa_gpu_context *ctx, double tol);
+
 /* fe11, fe22, fe33 ((2Nx+1)(2Ny+1) square, column-major) of one permittivity
  * grid eps[(n_cy-1)][(n_cx-1)] on the given edges (Li rules of
  * rcwa/FourierSolver2D.cpp:12-351). */

@@ -2,9 +2,9 @@
 
 Modules:
   engine     contexts, plans and batched solves over include/openrcwa_gpu.h
-  solver     RCWASolver / Layer facade with the reference's method names
   ofd        batched host logic of the .ofd RCWA path
+  orcwa      the .orcwa input path (parser, driver, orcwa_rcwa CSV command)
   workloads  seeded synthetic batches
   build      hipcc build of libopenrcwa_gpu.so
 """
-__all__ = ["engine", "solver", "ofd", "workloads"]
+__all__ = ["engine", "ofd", "orcwa", "workloads"]

@@ -20,6 +20,8 @@ STATUS_EIG_RESIDUAL = 1
 STATUS_GRAZING = 2
 STATUS_SINGULAR = 4
 STATUS_NOCONV = 8
+STATUS_EIG_PERTURBED = 16
+STATUS_ILLCOND = 32
 
 
 class OrcwaStack(ctypes.Structure):
@@ -72,7 +74,8 @@ SLICES = {"xy": 0, "xz": 1, "yz": 2}
 
 
 # every symbol include/openrcwa_gpu.h declares
-EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_solve_batch",
+EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_set_workspace_limit",
+            "orcwa_gpu_solve_batch",
             "orcwa_plan_create", "orcwa_plan_execute", "orcwa_plan_sync",
             "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms",
             "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "orcwa_gpu_field_image",
@@ -96,6 +99,7 @@ def load(path: str | None = None):
     lib = ctypes.CDLL(p)
     lib.orcwa_gpu_create.argtypes = [POINTER(c_void_p), c_int]
     lib.orcwa_gpu_destroy.argtypes = [c_void_p]
+    lib.orcwa_gpu_set_workspace_limit.argtypes = [c_void_p, ctypes.c_size_t]
     lib.orcwa_gpu_solve_batch.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
                                           c_int, POINTER(OrcwaResult)]
     lib.orcwa_plan_create.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),

@@ -22,8 +22,9 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unknown-warning-option"]
 
+# .hip files hold device code; .cpp files are host-only C++ (no HIP headers)
 SOURCES = ["blas.hip", "eig.hip", "rcwa_kernels.hip", "field_kernels.hip", "plan.hip",
-           "smatrix.hip", "debug.hip", "bridge.hip"]
+           "smatrix.hip", "debug.hip", "ofd_bridge.cpp"]
 
 
 def _run(cmd, verbose):
@@ -44,7 +45,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     objs = [os.path.join(BUILD_DIR, os.path.splitext(s)[0] + ".o") for s in srcs]
 
     def compile_one(i):
-        _run([HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, srcs[i]), "-o", objs[i]],
+        lang = "c++" if srcs[i].endswith(".cpp") else "hip"
+        _run([HIPCC, *FLAGS, "-x", lang, "-c", os.path.join(CSRC, srcs[i]), "-o", objs[i]],
              verbose)
 
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, len(srcs)))) as ex:

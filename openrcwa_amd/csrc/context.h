@@ -18,6 +18,15 @@ struct orcwa_gpu_context {
   };
   std::vector<Cached> cache;
   size_t cached_bytes = 0;
+  // workspace budget of this context's plans in bytes (0: 70 % of the free
+  // device memory); set by orcwa_gpu_set_workspace_limit
+  size_t workspace_limit = 0;
+  // plans created on this context and not yet destroyed: the context refuses
+  // to be destroyed while any is live
+  int live_plans = 0;
+  // residual acceptance of the eigen solves (orcwa_dbg_set_eig_tolerance;
+  // 1e-8 as rcwa/MKLEigenSolver.hpp:75-79)
+  double eig_tol = 1e-8;
 };
 
 namespace orcwa {

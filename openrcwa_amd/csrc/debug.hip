@@ -130,7 +130,7 @@ int orcwa_dbg_zgeev(orcwa_gpu_context* ctx, int n, const double* A, double* lam,
   ws.X = x.table;
   ws.tau = dtau;
   zgeev_batched(s, n, a.table, (const cd* const*)ao.table, w.table, dlam, ws, dinfo, dres,
-                x.table, batch, nullptr);
+                x.table, batch, nullptr, ctx->eig_tol);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(lam, dlam, sizeof(cd) * n * batch, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(W, w.data, sizeof(cd) * nn * batch, hipMemcpyDeviceToHost, s);
@@ -142,6 +142,12 @@ int orcwa_dbg_zgeev(orcwa_gpu_context* ctx, int n, const double* A, double* lam,
   (void)hipFree(dinfo);
   (void)hipFree(dres);
   DCHK(e);
+  return ORCWA_OK;
+}
+
+int orcwa_dbg_set_eig_tolerance(orcwa_gpu_context* ctx, double tol) {
+  if (!ctx || tol != tol) return ORCWA_ERR_ARG;
+  ctx->eig_tol = tol;
   return ORCWA_OK;
 }
 

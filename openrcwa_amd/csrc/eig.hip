@@ -500,124 +500,6 @@ __device__ inline bool small_subdiag(const cd* H, int n, int k, int lo, int hi, 
   return false;
 }
 
-__device__ inline cd wilkinson_shift(const cd* H, int n, int hi) {
-  cd t = H[hi + (long long)hi * n];
-  cd u = csqrt_principal(H[(hi - 1) + (long long)hi * n]) *
-         csqrt_principal(H[hi + (long long)(hi - 1) * n]);
-  double s = cabs1(u);
-  if (s != 0.0) {
-    cd x = 0.5 * (H[(hi - 1) + (long long)(hi - 1) * n] - t);
-    double sx = cabs1(x);
-    s = fmax(s, sx);
-    cd xs = (1.0 / s) * x, us = (1.0 / s) * u;
-    cd y = s * csqrt_principal(xs * xs + us * us);
-    if (sx > 0.0) {
-      cd xx = (1.0 / sx) * x;
-      if (xx.re * y.re + xx.im * y.im < 0.0) y = -y;
-    }
-    t = t - u * cdiv(u, x + y);
-  }
-  return t;
-}
-
-__global__ __launch_bounds__(256) void k_qr_simple(int n, cd* const* Hp, cd* const* Zp,
-                                                   int* info, int maxit) {
-  __shared__ int s_l;
-  __shared__ cd s_c, s_s;
-  __shared__ int s_red[256];
-  const int b = blockIdx.x;
-  cd* __restrict__ H = Hp[b];
-  cd* __restrict__ Z = Zp[b];
-  const int tid = threadIdx.x;
-  const double smlnum = kSafmin * ((double)n / kUlp);
-
-  int hi = n - 1, its = 0, total = 0;
-  while (hi > 0) {
-    // largest k in [1, hi] whose subdiagonal is negligible (0 if none)
-    int best = 0;
-    for (int k = 1 + tid; k <= hi; k += 256)
-      if (small_subdiag(H, n, k, 0, hi, smlnum)) best = max(best, k);
-    s_red[tid] = best;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (tid < st) s_red[tid] = max(s_red[tid], s_red[tid + st]);
-      __syncthreads();
-    }
-    const int l = s_red[0];
-    __syncthreads();
-    if (l > 0 && tid == 0) H[l + (long long)(l - 1) * n] = mk(0.0, 0.0);
-    if (l == hi) {
-      --hi;
-      its = 0;
-      __syncthreads();
-      continue;
-    }
-    ++its;
-    ++total;
-    if (total > maxit) {
-      if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV;
-      break;
-    }
-    cd shift;
-    if (its % 30 == 10) {
-      shift = mk(0.75 * cabs1(H[(l + 1) + (long long)l * n]), 0.0) + H[l + (long long)l * n];
-    } else if (its % 30 == 20) {
-      shift = mk(0.75 * cabs1(H[hi + (long long)(hi - 1) * n]), 0.0) + H[hi + (long long)hi * n];
-    } else {
-      shift = wilkinson_shift(H, n, hi);
-    }
-    __syncthreads();
-    for (int k = l; k < hi; ++k) {
-      if (tid == 0) {
-        cd x0, x1;
-        if (k == l) {
-          x0 = H[l + (long long)l * n] - shift;
-          x1 = H[(l + 1) + (long long)l * n];
-        } else {
-          x0 = H[k + (long long)(k - 1) * n];
-          x1 = H[(k + 1) + (long long)(k - 1) * n];
-        }
-        const double nr = hypot(cabs(x0), cabs(x1));
-        if (nr == 0.0) {
-          s_c = mk(1.0, 0.0);
-          s_s = mk(0.0, 0.0);
-        } else {
-          s_c = (1.0 / nr) * x0;
-          s_s = (1.0 / nr) * x1;
-          if (k > l) {
-            H[k + (long long)(k - 1) * n] = mk(nr, 0.0);
-            H[(k + 1) + (long long)(k - 1) * n] = mk(0.0, 0.0);
-          }
-        }
-      }
-      __syncthreads();
-      const cd c = s_c, s = s_s;
-      // rows k, k+1 from the left (columns k..n-1; column k-1 set above)
-      for (int j = k + tid; j < n; j += 256) {
-        const long long cj = (long long)j * n;
-        const cd a = H[k + cj], bb = H[k + 1 + cj];
-        H[k + cj] = cmul(c, a) + cmul(s, bb);
-        H[k + 1 + cj] = c * bb - s * a;
-      }
-      __syncthreads();
-      // columns k, k+1 from the right (rows 0..min(k+2,hi)), and Z
-      const int iend = min(k + 2, hi);
-      const long long ck = (long long)k * n, ck1 = (long long)(k + 1) * n;
-      for (int i = tid; i <= iend; i += 256) {
-        const cd p = H[i + ck], q = H[i + ck1];
-        H[i + ck] = p * c + q * s;
-        H[i + ck1] = q * conj(c) - p * conj(s);
-      }
-      for (int i = tid; i < n; i += 256) {
-        const cd p = Z[i + ck], q = Z[i + ck1];
-        Z[i + ck] = p * c + q * s;
-        Z[i + ck1] = q * conj(c) - p * conj(s);
-      }
-      __syncthreads();
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // 3b. windowed small-bulge multishift QR (one workgroup per matrix).
 //
@@ -632,19 +514,6 @@ __global__ __launch_bounds__(256) void k_qr_simple(int n, cd* const* Hp, cd* con
 // rows are finished entirely in LDS.  Shifts are the eigenvalues of the
 // trailing QNS x QNS submatrix.
 // ---------------------------------------------------------------------------
-#ifdef ORCWA_QR_PROFILE
-// per-phase shader-clock totals of matrix 0 (profiling builds only)
-__device__ unsigned long long g_qr_prof[24];
-#define QPROF_T0(v) unsigned long long v = (threadIdx.x == 0 && blockIdx.x == 0) ? clock64() : 0
-#define QPROF_ADD(slot, v) \
-  if (threadIdx.x == 0 && blockIdx.x == 0) g_qr_prof[slot] += clock64() - (v)
-#define QPROF_CNT(slot) \
-  if (threadIdx.x == 0 && blockIdx.x == 0) g_qr_prof[slot] += 1
-#else
-#define QPROF_T0(v)
-#define QPROF_ADD(slot, v)
-#define QPROF_CNT(slot)
-#endif
 
 // A 48-wide window with 12 shifts keeps the LDS of one workgroup under 80 KB,
 // so two matrices run on each CU and one's latency-bound chase and window
@@ -751,28 +620,19 @@ __device__ __forceinline__ bool wave_small_qr(cd* B, cd* U, int m, bool wantu, i
   const double smlnum = kSafmin * ((double)m / kUlp);
   int hi = m - 1, its = 0, tot = 0;
   while (hi > 0) {
-    QPROF_T0(ts0);
     const bool flag = (lane >= 1 && lane <= hi) ? small_subdiag_lds(B, lane, 0, hi, smlnum) : false;
     const unsigned long long mask = __ballot(flag);
     const int l = mask ? 63 - __clzll(mask) : 0;
     wave_fence();
     if (l > 0 && lane == 0) B[l + (l - 1) * QLD] = mk(0.0, 0.0);
     wave_fence();
-    QPROF_ADD(19, ts0);
     if (l == hi) { --hi; its = 0; continue; }
     ++its;
     if (++tot > maxit) return false;
-    QPROF_T0(ts1);
-    QPROF_CNT(22);
     cd sh;
     if (its % 30 == 10) sh = mk(0.75 * cabs1(B[(l + 1) + l * QLD]), 0.0) + B[l + l * QLD];
     else if (its % 30 == 20) sh = mk(0.75 * cabs1(B[hi + (hi - 1) * QLD]), 0.0) + B[hi + hi * QLD];
     else sh = wilkinson_lds(B, hi);
-    QPROF_ADD(20, ts1);
-    QPROF_T0(ts2);
-#ifdef ORCWA_QR_PROFILE
-    if (threadIdx.x == 0 && blockIdx.x == 0) g_qr_prof[23] += hi - l;
-#endif
     for (int k = l; k < hi; ++k) {
       cd x0, x1;
       if (k == l) { x0 = B[l + l * QLD] - sh; x1 = B[(l + 1) + l * QLD]; }
@@ -807,7 +667,6 @@ __device__ __forceinline__ bool wave_small_qr(cd* B, cd* U, int m, bool wantu, i
       }
       wave_fence();
     }
-    QPROF_ADD(21, ts2);
   }
   return true;
 }
@@ -1026,10 +885,7 @@ __device__ int aed_window(QrShared& S, cd* H, cd* Z, int n, int hi, int nw, doub
   const int kw = hi - nw + 1;
   const int tid = threadIdx.x, lane = tid & 63;
   ms_load_window(S, H, n, kw, nw);
-  QPROF_T0(tq0);
   if (!lds_small_qr(S, S.Hw, nw, true, 30 * nw + 100)) return -1;
-  QPROF_ADD(15, tq0);
-  QPROF_T0(tq1);
   const cd s = H[kw + (long long)(kw - 1) * n];
   cd* T = S.Hw;
   cd* V = S.U;
@@ -1070,7 +926,6 @@ __device__ int aed_window(QrShared& S, cd* H, cd* Z, int n, int hi, int nw, doub
     if (lane == 0) S.ok = (ns >= QNS) ? 1 : 0;
   }
   __syncthreads();
-  QPROF_ADD(7, tq1);
   const int code = S.nsched;
   if (code < 0) return 0;
   ms_store_window(S, H, n, kw, nw);
@@ -1094,7 +949,6 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
   const double smlnum = kSafmin * ((double)n / kUlp);
   int hi = n - 1, sweeps = 0, noprog = 0;
   while (hi >= 0) {
-    QPROF_T0(tp0);
     int best = 0;
     for (int k = 1 + tid; k <= hi; k += 256)
       if (small_subdiag(H, n, k, 0, hi, smlnum)) best = max(best, k);
@@ -1105,12 +959,9 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
     __syncthreads();
     if (l > 0 && tid == 0) H[l + (long long)(l - 1) * n] = mk(0.0, 0.0);
     __syncthreads();
-    QPROF_ADD(0, tp0);
     if (l == hi) { --hi; noprog = 0; continue; }
     const int m = hi - l + 1;
     if (m <= QW) {
-      QPROF_T0(tp1);
-      QPROF_CNT(8);
       ms_load_window(S, H, n, l, m);
       __syncthreads();
       const bool ok = lds_small_qr(S, S.Hw, m, true, 30 * m + 100);
@@ -1118,7 +969,6 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
       __syncthreads();
       ms_apply_window(S, H, Z, n, l, m);
       __syncthreads();
-      QPROF_ADD(1, tp1);
       if (!ok) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
       hi = l - 1;
       noprog = 0;
@@ -1128,13 +978,7 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
     // unless it deflated more than 14 % of the window
     bool aed_shifts = false;
     {
-      QPROF_T0(tpa);
       const int nd = aed_window(S, H, Z, n, hi, AED_NW, smlnum);
-      QPROF_ADD(14, tpa);
-      QPROF_CNT(12);
-#ifdef ORCWA_QR_PROFILE
-      if (threadIdx.x == 0 && blockIdx.x == 0 && nd > 0) g_qr_prof[13] += nd;
-#endif
       if (nd > 0) {
         hi -= nd;
         noprog = 0;
@@ -1142,7 +986,6 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
       }
       aed_shifts = (nd >= 0) && (S.ok != 0);
     }
-    QPROF_T0(tp2);
     // shifts: the bottom undeflated eigenvalues of the early-deflation
     // window, else the eigenvalues of the trailing QNS x QNS block (in S.U)
     constexpr int ns = QNS;
@@ -1171,20 +1014,12 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
       S.bdone[tid] = 0;
     }
     __syncthreads();
-    QPROF_ADD(2, tp2);
-    QPROF_CNT(9);
     if (++sweeps > maxsweep) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
     int a = l;
     while (true) {
-      QPROF_T0(tp3);
-      QPROF_CNT(10);
       const int bend = min(a + QW, hi + 1);
       const int wa = bend - a;
       ms_load_window(S, H, n, a, wa);
-#ifdef ORCWA_QR_PROFILE
-      __syncthreads();
-      QPROF_ADD(6, tp3);
-#endif
       if (tid < 64) {
         // Schedule of (bulge, pair) moves inside this window, grouped in
         // passes (the moves of one pass touch disjoint row and column pairs).
@@ -1238,12 +1073,9 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
         }
       }
       __syncthreads();
-      QPROF_ADD(3, tp3);
-      QPROF_T0(tp4);
       const int cnt = S.nsched, npass = S.npass;
       const int rlim = min(hi - a + 1, wa);
       for (int ps = 0; ps < npass; ++ps) {
-        QPROF_CNT(11);
         const int e0 = S.pass_start[ps], ne = S.pass_start[ps + 1] - e0;
         if (tid < ne) {
           const int e = e0 + tid;
@@ -1295,12 +1127,9 @@ __global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* co
         }
         __syncthreads();
       }
-      QPROF_ADD(4, tp4);
-      QPROF_T0(tp5);
       ms_store_window(S, H, n, a, wa);
       ms_apply_window(S, H, Z, n, a, wa);
       __syncthreads();
-      QPROF_ADD(5, tp5);
       const int next = S.l;
       __syncthreads();
       if (next == (1 << 30)) break;
@@ -1428,10 +1257,12 @@ __global__ __launch_bounds__(256) void k_eig_finish(int n, const cd* const* Tp, 
   }
 }
 
-// R = A W already in Rp; residual = max|R - W diag(lam)| / max|A|
+// R = A W already in Rp; residual = max|R - W diag(lam)| / max|A|, accepted
+// when <= tol (1e-8 in the reference, rcwa/MKLEigenSolver.hpp:75-79)
 __global__ __launch_bounds__(256) void k_eig_residual(int n, const cd* const* Ap,
                                                       const cd* const* Wp, const cd* const* Rp,
-                                                      const cd* lam, double* resid, int* info) {
+                                                      const cd* lam, double* resid, int* info,
+                                                      double tol) {
   __shared__ double sred[4];
   const int b = blockIdx.x;
   const cd* A = Ap[b];
@@ -1458,7 +1289,7 @@ __global__ __launch_bounds__(256) void k_eig_residual(int n, const cd* const* Ap
     const double r = fmax(fmax(sred[0], sred[1]), fmax(sred[2], sred[3]));
     const double rel = (a > 0.0) ? r / a : 0.0;
     resid[b] = rel;
-    if (!(rel <= 1e-8)) info[b] |= ORCWA_STATUS_EIG_RESIDUAL;
+    if (!(rel <= tol)) info[b] |= ORCWA_STATUS_EIG_RESIDUAL;
   }
 }
 
@@ -1472,20 +1303,63 @@ __global__ void k_zero_below_subdiag(int n, cd* const* Ap) {
   }
 }
 
-#ifdef ORCWA_QR_PROFILE
-extern "C" int orcwa_prof_qr(double* out) {
-  unsigned long long h[24];
-  if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_qr_prof), sizeof(h)) != hipSuccess) return 1;
-  for (int i = 0; i < 24; ++i) out[i] = (double)h[i];
-  unsigned long long z[24] = {0};
-  if (hipMemcpyToSymbol(HIP_SYMBOL(g_qr_prof), z, sizeof(z)) != hipSuccess) return 1;
-  return 0;
+
+// The perturbed matrix of the reference's recovery solve
+// (rcwa/MKLEigenSolver.hpp:81-93): A = Aorig + diag(1e-11 (i+1) max|Aorig|),
+// one workgroup per matrix.
+__global__ __launch_bounds__(256) void k_perturb_diag(int n, const cd* const* Aop,
+                                                      cd* const* Ap) {
+  __shared__ double sm[4];
+  const cd* Ao = Aop[blockIdx.x];
+  cd* A = Ap[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long long nn = (long long)n * n;
+  double m = 0.0;
+  for (long long e = tid; e < nn; e += 256) m = fmax(m, cabs(Ao[e]));
+  m = wave_max(m);
+  if (lane == 0) sm[wave] = m;
+  __syncthreads();
+  const double scale = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+  for (long long e = tid; e < nn; e += 256) {
+    cd v = Ao[e];
+    const long long i = e % n, j = e / n;
+    if (i == j) v.re += scale * 1e-11 * (double)(i + 1);
+    A[e] = v;
+  }
 }
-#endif
+
+void eig_perturb_batched(hipStream_t s, int n, const cd* const* Aorig, cd* const* A, int batch) {
+  if (batch <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_perturb_diag, dim3(batch), dim3(256), 0, s, n, Aorig, A);
+}
+
+// results of the recovery solves back into the slots of their problems:
+// problem idx[k] gets lam_k, status info_k | EIG_PERTURBED (replacing the
+// failed solve's bits) and residual res_k
+__global__ __launch_bounds__(256) void k_eig_recover_scatter(int n, const int* idx,
+                                                             const cd* lam_k, const int* info_k,
+                                                             const double* res_k, cd* lam,
+                                                             int* info, double* resid) {
+  const int k = blockIdx.x, q = idx[k];
+  for (int j = threadIdx.x; j < n; j += 256)
+    lam[(long long)q * n + j] = lam_k[(long long)k * n + j];
+  if (threadIdx.x == 0) {
+    info[q] = info_k[k] | ORCWA_STATUS_EIG_PERTURBED;
+    resid[q] = res_k[k];
+  }
+}
+
+void eig_recover_scatter(hipStream_t s, int n, int m, const int* idx, const cd* lam_k,
+                         const int* info_k, const double* res_k, cd* lam, int* info,
+                         double* resid) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_eig_recover_scatter, dim3(m), dim3(256), 0, s, n, idx, lam_k, info_k,
+                     res_k, lam, info, resid);
+}
 
 void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, cd* const* W,
                    cd* lam, const EigWorkspace& ws, int* info, double* resid,
-                   cd* const* scratch, int batch, hipEvent_t* ev) {
+                   cd* const* scratch, int batch, hipEvent_t* ev, double tol) {
   if (batch <= 0 || n <= 0) return;
   const cd* const* Ac = (const cd* const*)A;
   if (n >= 3) {
@@ -1543,7 +1417,7 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
                   mk(0.0, 0.0), scratch, 0, 1, n, batch);
     hipLaunchKernelGGL(k_eig_residual, dim3(batch), dim3(256), 0, s, n, Aorig,
                        (const cd* const*)W, (const cd* const*)scratch, (const cd*)lam, resid,
-                       info);
+                       info, tol);
   }
   if (ev) (void)hipEventRecord(ev[2], s);
 }

@@ -12,6 +12,9 @@
 
 namespace orcwa {
 
+// residual acceptance of an eigen solve (rcwa/MKLEigenSolver.hpp:75-79)
+constexpr double kEigTol = 1e-8;
+
 // Batched complex GEMM on FP64 MFMA:
 //   C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]
 // op(X)[i,k] = X[off + i*rs + k*cs], conjugated when conj != 0.
@@ -45,8 +48,10 @@ void copy_batched(hipStream_t s, long long count, const cd* const* A, cd* const*
 // Eigensolver for a batch of general complex n x n matrices (column-major):
 // A[b] is overwritten (Schur form).  On exit W[b] holds unit-2-norm
 // eigenvectors and lam[b*n + j] the eigenvalues.  work must hold 2 n*n per
-// matrix (Z and X).  Aorig (may equal nullptr) is used for the residual test;
-// status bits are or-ed into info[b].  resid[b] receives the relative residual.
+// matrix (Z and X).  Aorig (may equal nullptr) is used for the residual test
+// (accepted when ||Aorig W - W diag(lam)||max / ||Aorig||max <= tol, else
+// ORCWA_STATUS_EIG_RESIDUAL); status bits are or-ed into info[b].  resid[b]
+// receives the relative residual.
 struct EigWorkspace {
   cd** Z;        // [batch] n*n
   cd** X;        // [batch] n*n
@@ -54,7 +59,17 @@ struct EigWorkspace {
 };
 void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, cd* const* W,
                    cd* lam, const EigWorkspace& ws, int* info, double* resid,
-                   cd* const* scratch, int batch, hipEvent_t* phase_events);
+                   cd* const* scratch, int batch, hipEvent_t* phase_events,
+                   double tol = kEigTol);
+// A[b] = Aorig[b] + diag(1e-11 (i+1) max|Aorig[b]|): the matrix of the
+// reference's recovery solve after a failed residual test
+// (rcwa/MKLEigenSolver.hpp:81-93).
+void eig_perturb_batched(hipStream_t s, int n, const cd* const* Aorig, cd* const* A, int batch);
+// For k < m: lam[idx[k]] (n values) = lam_k[k], info[idx[k]] = info_k[k] |
+// ORCWA_STATUS_EIG_PERTURBED, resid[idx[k]] = res_k[k].
+void eig_recover_scatter(hipStream_t s, int n, int m, const int* idx, const cd* lam_k,
+                         const int* info_k, const double* res_k, cd* lam, int* info,
+                         double* resid);
 
 // Modes (W, V, keff in k0 units) of every layer of one instance, each layer
 // with its own cell edges layer_cx [L][n_cx], layer_cy [L][n_cy] (same

@@ -78,6 +78,10 @@ struct ChunkTables {
 
 struct orcwa_plan {
   orcwa_gpu_context* ctx = nullptr;
+  // every exit of a plan, failed creations included, goes through here
+  ~orcwa_plan() {
+    if (ctx) --ctx->live_plans;
+  }
   int Nx = 0, Ny = 0, nx = 0, ny = 0, h = 0, n = 0, L = 0, count = 0;
   int ncx = 0, ncy = 0, ncell = 0, maxpol = 1;
   double Lx = 0, Ly = 0;
@@ -282,8 +286,16 @@ int orcwa_gpu_create(orcwa_gpu_context** ctx, int device) {
   return ORCWA_OK;
 }
 
+int orcwa_gpu_set_workspace_limit(orcwa_gpu_context* ctx, size_t bytes) {
+  if (!ctx) return fail("orcwa_gpu_set_workspace_limit: null context");
+  ctx->workspace_limit = bytes;
+  return ORCWA_OK;
+}
+
 int orcwa_gpu_destroy(orcwa_gpu_context* ctx) {
   if (!ctx) return ORCWA_OK;
+  if (ctx->live_plans > 0)
+    return fail("orcwa_gpu_destroy: the context still has live plans (destroy them first)");
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   ctx_trim(ctx);
@@ -345,6 +357,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   hipSetDevice(ctx->device);
   orcwa_plan* p = new orcwa_plan;
   p->ctx = ctx;
+  ++ctx->live_plans;
   p->keep = keep;
   p->modes_only = modes_only;
   p->keep_pq = keep_pq && modes_only && keep;
@@ -524,11 +537,13 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
     lp_max = std::max(lp_max, a);
     lu_max = std::max(lu_max, b);
   }
-  // workspace budget: ORCWA_WORKSPACE_GB, else 70% of the device memory free
-  // now (the per-instance slots dominate; a few large chunks keep every
-  // batched kernel wide and pay the end-of-launch tail few times)
-  double budget_gb = 0.0;
-  if (const char* s = getenv("ORCWA_WORKSPACE_GB")) budget_gb = atof(s);
+  // workspace budget: the context's limit (orcwa_gpu_set_workspace_limit),
+  // else ORCWA_WORKSPACE_GB, else 70% of the device memory free now (the
+  // per-instance slots dominate; a few large chunks keep every batched kernel
+  // wide and pay the end-of-launch tail few times)
+  double budget_gb = (double)ctx->workspace_limit / 1e9;
+  if (!(budget_gb > 0.0))
+    if (const char* s = getenv("ORCWA_WORKSPACE_GB")) budget_gb = atof(s);
   if (!(budget_gb > 0.0)) {
     size_t fr = 0, tot = 0;
     // (buffers cached by the context count as free)
@@ -983,6 +998,71 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
   return plan_create(ctx, st, inst, count, false, out);
 }
 
+// The reference's recovery of a failed eigen solve
+// (rcwa/MKLEigenSolver.hpp:160-176): problems of the chunk whose residual
+// test failed (or whose QR did not converge) are solved again on
+// A + diag(1e-11 (i+1) max|A|) and tested against the unperturbed A.  The
+// status bits are read back once per chunk; the recovery itself is rare.
+static int eig_recover(orcwa_plan* p, const ChunkTables& ct, hipStream_t s) {
+  const int np = ct.nprob, n = p->n;
+  if (np <= 0) return ORCWA_OK;
+  std::vector<int> pin(np);
+  HIPCHK(hipMemcpyAsync(pin.data(), p->d_pinfo + ct.prob_base, np * sizeof(int),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  std::vector<int> bad;
+  for (int q = 0; q < np; ++q)
+    if (pin[q] & (ORCWA_STATUS_EIG_RESIDUAL | ORCWA_STATUS_NOCONV)) bad.push_back(q);
+  if (bad.empty()) return ORCWA_OK;
+  const int m = (int)bad.size();
+  // the problems' slots, read from the host image of the plan's tables
+  auto slot = [&](size_t off, int q) {
+    return reinterpret_cast<cd* const*>(p->blob.host.data() + off)[q];
+  };
+  std::vector<cd*> tabs((size_t)5 * m);
+  for (int k = 0; k < m; ++k) {
+    tabs[k] = slot(ct.pA, bad[k]);
+    tabs[m + k] = slot(ct.pAorig, bad[k]);
+    tabs[2 * m + k] = slot(ct.pW, bad[k]);
+    tabs[3 * m + k] = slot(ct.pZ, bad[k]);
+    tabs[4 * m + k] = slot(ct.pX, bad[k]);
+  }
+  const size_t o_idx = align256(tabs.size() * sizeof(cd*));
+  const size_t o_lam = align256(o_idx + m * sizeof(int));
+  const size_t o_info = align256(o_lam + (size_t)m * n * sizeof(cd));
+  const size_t o_res = align256(o_info + m * sizeof(int));
+  const size_t bytes = o_res + m * sizeof(double);
+  char* buf = nullptr;
+  if (hipMalloc(&buf, bytes) != hipSuccess) return ORCWA_ERR_NOMEM;
+  std::vector<char> img(o_lam, 0);
+  std::memcpy(img.data(), tabs.data(), tabs.size() * sizeof(cd*));
+  std::memcpy(img.data() + o_idx, bad.data(), m * sizeof(int));
+  hipError_t e = hipMemcpyAsync(buf, img.data(), img.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(buf + o_info, 0, m * sizeof(int), s);
+  if (e == hipSuccess) {
+    cd* const* tA = reinterpret_cast<cd* const*>(buf);
+    const cd* const* tAo = reinterpret_cast<const cd* const*>(buf) + m;
+    cd* const* tW = reinterpret_cast<cd* const*>(buf) + 2 * m;
+    EigWorkspace ws;
+    ws.Z = reinterpret_cast<cd**>(buf) + 3 * m;
+    ws.X = reinterpret_cast<cd**>(buf) + 4 * m;
+    ws.tau = p->d_tau;
+    cd* lam_k = reinterpret_cast<cd*>(buf + o_lam);
+    int* info_k = reinterpret_cast<int*>(buf + o_info);
+    double* res_k = reinterpret_cast<double*>(buf + o_res);
+    eig_perturb_batched(s, n, tAo, tA, m);
+    zgeev_batched(s, n, tA, tAo, tW, lam_k, ws, info_k, res_k, ws.X, m, nullptr,
+                  p->ctx->eig_tol);
+    eig_recover_scatter(s, n, m, reinterpret_cast<const int*>(buf + o_idx), lam_k, info_k,
+                        res_k, p->d_lam, p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  HIPCHK(e);
+  return ORCWA_OK;
+}
+
 int orcwa_plan_execute(orcwa_plan* p) {
   if (!p) return fail("null plan");
   HIPCHK(hipSetDevice(p->ctx->device));
@@ -1060,7 +1140,11 @@ int orcwa_plan_execute(orcwa_plan* p) {
       HIPCHK(hipEventRecord(p->ev[5], s));
       zgeev_batched(s, n, A, p->dev<const cd* const*>(ct.pAorig), Wp, p->d_lam, ws,
                     p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base,
-                    p->dev<cd* const*>(ct.pX), np, nullptr);
+                    p->dev<cd* const*>(ct.pX), np, nullptr, p->ctx->eig_tol);
+      {
+        const int rc = eig_recover(p, ct, s);
+        if (rc != ORCWA_OK) return rc;
+      }
       HIPCHK(hipEventRecord(p->ev[6], s));
       // QW1 = [E22 W_bottom ; E11 W_top]
       cd* const* QW = p->dev<cd* const*>(ct.pQW);
@@ -1154,7 +1238,7 @@ int orcwa_plan_execute(orcwa_plan* p) {
       cd* const* Rnext = p->dev<cd* const*>(ct.Rt[(si + 1) & 1]);
       if (si > 0)
         launch_phase_scale(s, n, p->dev<const cd* const*>(st.keff_b),
-                           p->dev<const double*>(st.k0d), Rcur, nullptr, ni);
+                           p->dev<const double*>(st.k0d), Rcur, nullptr, p->d_info + ct.i0, ni);
       // M^T = (F + G)^T = A^T + (B R~)^T; the GEMM reads R~ through R~^T and
       // stores its product transposed
       launch_transpose(s, n, C(E), M, ni);

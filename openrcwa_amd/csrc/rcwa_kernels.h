@@ -122,7 +122,7 @@ void launch_uniform(hipStream_t s, const ConvGeom& g, const UniParam* params, cd
 void launch_bdinv_apply(hipStream_t s, int h, const cd* const* binv, const cd* const* Vb,
                         cd* const* H, int batch);
 void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const double* k0d,
-                        cd* const* R, cd* const* T, int batch);
+                        cd* const* R, cd* const* T, int* info, int batch);
 void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* rt,
                double* ref, double* trn, cd* amp, int* info, int max_pol, int batch);
 void launch_transpose(hipStream_t s, int n, const cd* const* src, cd* const* dst, int batch);

@@ -415,8 +415,13 @@ __global__ void k_bdinv_apply(int h, const cd* const* binv, const cd* const* Vb,
 
 // R <- Phi R Phi, T <- T Phi with phi = exp(i keff (k0 d))
 // (rcwa/RCWASolver.cpp:415-434)
+// A mode that grows by more than this factor across one layer (Im keff < 0
+// under the GammaSqrtOp branch of rcwa/WaveEquationSolver.cpp:13-21) makes
+// the cascade amplify rounding by about its square: ORCWA_STATUS_ILLCOND.
+__device__ constexpr double kGrowthLog = 9.2103403719761836;   // ln(1e4)
+
 __global__ void k_phase_scale(int n, const cd* const* keff, const double* k0d, cd* const* R,
-                              cd* const* T) {
+                              cd* const* T, int* info) {
   const int b = blockIdx.y;
   const long long nn = (long long)n * n;
   const double kd = k0d[b];
@@ -425,6 +430,7 @@ __global__ void k_phase_scale(int n, const cd* const* keff, const double* k0d, c
        e += (long long)gridDim.x * blockDim.x) {
     const int r = (int)(e % n), c = (int)(e / n);
     const cd ar = kd * kf[r], ac = kd * kf[c];
+    if (c == 0 && info && -ar.im > kGrowthLog) atomicOr(&info[b], ORCWA_STATUS_ILLCOND);
     const cd pr = cexp(mk(-ar.im, ar.re));
     const cd pc = cexp(mk(-ac.im, ac.re));
     R[b][e] = pr * R[b][e] * pc;
@@ -914,11 +920,11 @@ void launch_bdinv_apply(hipStream_t s, int h, const cd* const* binv, const cd* c
 }
 
 void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const double* k0d,
-                        cd* const* R, cd* const* T, int batch) {
+                        cd* const* R, cd* const* T, int* info, int batch) {
   if (batch <= 0) return;
   long long b = blocks_for((long long)n * n);
   hipLaunchKernelGGL(k_phase_scale, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s,
-                     n, keff, k0d, R, T);
+                     n, keff, k0d, R, T, info);
 }
 
 void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* rt,

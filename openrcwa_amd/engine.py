@@ -9,6 +9,7 @@ sol/rcwa_bridge.cpp:15-16; micrometres on the .orcwa path).
 from __future__ import annotations
 
 import ctypes
+import weakref
 from ctypes import POINTER, c_double, c_void_p
 from dataclasses import dataclass
 
@@ -27,17 +28,30 @@ class Context:
     def __init__(self, device: int = 0):
         self._lib = _lib.load()
         self.device = int(device)
+        self._plans = weakref.WeakSet()
         h = c_void_p()
         _lib.check(self._lib.orcwa_gpu_create(ctypes.byref(h), int(device)), "orcwa_gpu_create")
         self._h = h
 
     @property
     def handle(self):
+        if not getattr(self, "_h", None):
+            raise _lib.EngineError("the context is closed")
         return self._h
 
+    def set_workspace_limit(self, nbytes: int):
+        """Device-memory budget of the plans later created on this context
+        (orcwa_gpu_set_workspace_limit; 0 restores the default)."""
+        _lib.check(self._lib.orcwa_gpu_set_workspace_limit(self.handle, int(nbytes)),
+                   "orcwa_gpu_set_workspace_limit")
+
     def close(self):
+        """Destroy the context; its outstanding plans are closed first (the C
+        ABI refuses to destroy a context with live plans)."""
         if getattr(self, "_h", None):
-            self._lib.orcwa_gpu_destroy(self._h)
+            for p in list(self._plans):
+                p.close()
+            _lib.check(self._lib.orcwa_gpu_destroy(self._h), "orcwa_gpu_destroy")
             self._h = None
 
     def __del__(self):
@@ -149,6 +163,7 @@ class Plan:
         _lib.check(self._lib.orcwa_plan_create(ctx.handle, ctypes.byref(st), inst, B,
                                                ctypes.byref(h)), "orcwa_plan_create")
         self._h = h
+        ctx._plans.add(self)
 
     def execute(self):
         _lib.check(self._lib.orcwa_plan_execute(self._h), "orcwa_plan_execute")
@@ -209,6 +224,9 @@ def _uneven_groups(stacks, pad_layers=True):
     groups (by n^3 x layers x stacks) first."""
     groups = {}
     for i, s_ in enumerate(stacks):
+        if s_.get("mu") is not None and not np.all(np.asarray(s_["mu"]) == 1.0):
+            raise ValueError(f"stack {i}: magnetic layers (mu != 1) are not supported by "
+                             "solve_uneven; solve them with a Plan that takes mu")
         cx, cy = np.asarray(s_["coord_x"]), np.asarray(s_["coord_y"])
         key = (int(s_["nX"]), int(s_["nY"]), 0 if pad_layers else len(s_["thickness"]),
                float(cx[-1] - cx[0]), float(cy[-1] - cy[0]))
@@ -285,9 +303,9 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True
     status) in the input order.  With streams > 1 the groups are spread over
     that many contexts (one stream each, on the device of ctx) driven from as
     many host threads, so small groups share the GPU instead of running one
-    after another; their plans are then built one at a time, each sized to an
-    equal share of the device memory free at the start (unless
-    ORCWA_WORKSPACE_GB is set)."""
+    after another; their plans are then built one at a time, each context's
+    workspace limited to an equal share of the device memory free at the
+    start (unless ORCWA_WORKSPACE_GB is set)."""
     import os
     import queue
     import threading
@@ -321,6 +339,9 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True
         if free:
             share = 0.8 * free / streams / 1e9
     ctxs = [ctx] + [Context(ctx.device) for _ in range(streams - 1)]
+    if share is not None:
+        for c in ctxs:
+            c.set_workspace_limit(int(share * 1e9))
     errors = []
 
     def worker(c):
@@ -335,8 +356,6 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True
                 errors.append(e)
 
     threads = [threading.Thread(target=worker, args=(c,)) for c in ctxs]
-    if share is not None:
-        os.environ["ORCWA_WORKSPACE_GB"] = f"{share:.3f}"
     try:
         for t in threads:
             t.start()
@@ -344,7 +363,7 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True
             t.join()
     finally:
         if share is not None:
-            os.environ.pop("ORCWA_WORKSPACE_GB", None)
+            ctx.set_workspace_limit(0)
         for c in ctxs[1:]:
             c.close()
     if errors:

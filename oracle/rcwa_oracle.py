@@ -205,15 +205,22 @@ class EigenStats:
         self.max_residual = 0.0
 
 
+# residual acceptance of an eigen solve (rcwa/MKLEigenSolver.hpp:75-79); tests
+# lower it to force the recovery branch on ordinary matrices
+EIG_TOL = 1e-8
+
+
 def eig_backend(A, stats: EigenStats | None = None):
     """MKLEigenSolver<MatrixXcs>::compute (rcwa/MKLEigenSolver.hpp:103-179):
-    zgeev('N','V'); accept if the relative residual <= 1e-8, otherwise solve
-    A + diag(1e-11 (i+1) max|A|) (the Eigen fallback's perturbation, :81-93)."""
+    zgeev('N','V'); accept if the relative residual <= EIG_TOL (1e-8),
+    otherwise solve A + diag(1e-11 (i+1) max|A|) (the Eigen fallback's
+    perturbation, :81-93) and report its residual against the unperturbed A
+    (:170-176)."""
     d, V = np.linalg.eig(A)
     res = rel_residual(A, V, d)
     if stats is not None:
         stats.solves += 1
-    if not (res <= 1e-8):
+    if not (res <= EIG_TOL):
         scale = np.abs(A).max()
         Ap = A + np.diag(scale * 1e-11 * np.arange(1, A.shape[0] + 1))
         d, V = np.linalg.eig(Ap)

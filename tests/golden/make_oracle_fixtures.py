@@ -1,0 +1,97 @@
+"""Writes tests/golden/oracle_large.npz: per-order R/T of the FP64 oracle for
+instances too large to solve inside a GPU test run (the oracle's numpy zgeev
+at n = 1250 and 1922 takes minutes per stack).  The inputs are regenerated in
+the tests from the seeded workload generators, so the fixture holds only the
+instance descriptors and the oracle's outputs.
+
+  python tests/golden/make_oracle_fixtures.py      (CPU only, ~30 min)
+
+Contents (each REF/TRN array is (n_pol = 2, h), TE then TM):
+  pillar_idx, pillar_ref, pillar_trn    16 instances of the 2D 15 x 15-order
+                                        pillar batch (workloads.pillar2d())
+  uneven_seeds, uneven_n, uneven_ref... two 10-layer stacks of the uneven
+                                        generator at N = 11 and N = 12
+  large_*                               the 31 x 31-order grating with all six
+                                        layers (workloads.large2d()), frequency
+                                        index 20, theta = 20 deg, phi = 30 deg
+"""
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import rcwa_oracle as o  # noqa: E402
+from openrcwa_amd import workloads  # noqa: E402
+from openrcwa_amd.ofd import polarisation_basis  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle_large.npz")
+PILLAR_IDX = np.linspace(0, 2047, 16).astype(int)
+UNEVEN = [(11, 101), (12, 202)]      # (N, seed) of uneven_stacks(1, seed, (N, N), (10, 10))
+LARGE_FREQ, LARGE_THETA, LARGE_PHI = 20, 20.0, 30.0
+
+
+def uneven_stack(N, seed):
+    return workloads.uneven_stacks(1, seed=seed, n_range=(N, N), layer_range=(10, 10))[0]
+
+
+def large_instance():
+    w = workloads.large2d()
+    lam = w["lam"][LARGE_FREQ]
+    k0 = 2 * math.pi / lam
+    t, p = math.radians(LARGE_THETA), math.radians(LARGE_PHI)
+    te, tm = polarisation_basis(LARGE_THETA, LARGE_PHI)
+    return w, lam, k0 * math.sin(t) * math.cos(p), k0 * math.sin(t) * math.sin(p), (te, tm)
+
+
+def main():
+    out = {}
+    t0 = time.time()
+    w = workloads.pillar2d()
+    refs, trns = [], []
+    for i in PILLAR_IDX:
+        th, ph = w["angles"][i]
+        REF, TRN, _ = o.solve_stack(w["nX"], w["nY"], w["period"], w["period"], w["coord_x"],
+                                    w["coord_y"], list(w["eps"]), w["thickness"], w["lam"][i],
+                                    th, ph)
+        refs.append(REF)
+        trns.append(TRN)
+    out["pillar_idx"] = PILLAR_IDX
+    out["pillar_ref"] = np.array(refs)
+    out["pillar_trn"] = np.array(trns)
+    print(f"pillar: {time.time() - t0:.0f} s", flush=True)
+
+    refs, trns = [], []
+    for N, seed in UNEVEN:
+        s_ = uneven_stack(N, seed)
+        REF, TRN, _ = o.solve_stack(s_["nX"], s_["nY"], s_["period"], s_["period"],
+                                    s_["coord_x"], s_["coord_y"], list(s_["eps"]),
+                                    list(s_["thickness"]), s_["lam"], s_["theta"], s_["phi"],
+                                    pols=[tuple(p) for p in s_["pol"]])
+        refs.append(np.array(REF))
+        trns.append(np.array(TRN))
+        print(f"uneven N={N}: {time.time() - t0:.0f} s", flush=True)
+    out["uneven_n"] = np.array([u[0] for u in UNEVEN])
+    out["uneven_seeds"] = np.array([u[1] for u in UNEVEN])
+    for k, (r, t) in enumerate(zip(refs, trns)):
+        out[f"uneven_ref{k}"] = r
+        out[f"uneven_trn{k}"] = t
+
+    w, lam, kx0, ky0, pols = large_instance()
+    REF, TRN, _ = o.solve_stack(w["nX"], w["nY"], w["period"], w["period"], w["coord_x"],
+                                w["coord_y"], list(w["eps"]), list(w["thickness"]), lam,
+                                LARGE_THETA, LARGE_PHI, pols=list(pols))
+    out["large_freq_theta_phi"] = np.array([LARGE_FREQ, LARGE_THETA, LARGE_PHI])
+    out["large_ref"] = np.array(REF)
+    out["large_trn"] = np.array(TRN)
+    print(f"large: {time.time() - t0:.0f} s", flush=True)
+    np.savez_compressed(OUT, **out)
+    print(OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,6 +17,7 @@ def _lib_dbg():
                                     POINTER(c_double), POINTER(c_double), POINTER(c_int), c_int]
     lib.orcwa_dbg_zgeev.argtypes = [ctypes.c_void_p, c_int, POINTER(c_double), POINTER(c_double),
                                     POINTER(c_double), POINTER(c_double), POINTER(c_int), c_int]
+    lib.orcwa_dbg_set_eig_tolerance.argtypes = [ctypes.c_void_p, c_double]
     lib.orcwa_dbg_convolution.argtypes = [ctypes.c_void_p, c_int, c_int, c_int, c_int,
                                           POINTER(c_double), POINTER(c_double), POINTER(c_double),
                                           POINTER(c_double), POINTER(c_double), POINTER(c_double)]
@@ -96,3 +97,10 @@ def convolution(ctx, nX, nY, coord_x, coord_y, eps):
                                    _p(e.view(np.float64)), *[_p(o) for o in outs])
     _lib.check(rc, "orcwa_dbg_convolution")
     return outs
+
+
+def set_eig_tolerance(ctx, tol):
+    """Residual acceptance of later eigen solves on ctx (default 1e-8; a
+    negative value rejects every solve)."""
+    _lib.check(_lib_dbg().orcwa_dbg_set_eig_tolerance(ctx.handle, float(tol)),
+               "orcwa_dbg_set_eig_tolerance")

@@ -149,6 +149,30 @@ def test_zgeev_degenerate_block(gpu_ctx):
     assert np.linalg.cond(W[0]) < 1e6
 
 
+def test_zgeev_residual_status(gpu_ctx):
+    """k_eig_residual sets ORCWA_STATUS_EIG_RESIDUAL exactly for the matrices
+    whose relative residual ||AW - W diag(lam)||max / ||A||max exceeds the
+    acceptance tolerance (rcwa/MKLEigenSolver.hpp:66-79): with the tolerance
+    set between the residuals of a batch, the larger ones are flagged and
+    nothing else changes; a negative tolerance flags every matrix."""
+    from openrcwa_amd import _lib
+    A = crand(6, 40, 40)
+    _, W0, r0, info0 = g.zgeev(gpu_ctx, A)
+    assert not info0.any() and r0.max() < 1e-12
+    tol = float(np.median(r0))
+    try:
+        g.set_eig_tolerance(gpu_ctx, tol)
+        _, W1, r1, info1 = g.zgeev(gpu_ctx, A)
+        assert np.abs(r1 - r0).max() < 1e-15 and np.abs(W1 - W0).max() < 1e-13
+        want = np.where(r1 > tol, _lib.STATUS_EIG_RESIDUAL, 0)
+        assert np.array_equal(info1, want) and 0 < (want != 0).sum() < 6
+        g.set_eig_tolerance(gpu_ctx, -1.0)
+        _, _, _, info2 = g.zgeev(gpu_ctx, A)
+        assert (info2 == _lib.STATUS_EIG_RESIDUAL).all()
+    finally:
+        g.set_eig_tolerance(gpu_ctx, 1e-8)
+
+
 @pytest.mark.parametrize("nX,nY,xe,ye", [
     (5, 0, [0.0, 1.5e-7, 3e-7], [0.0, 3e-7]),
     (3, 2, [0.0, 1e-7, 2.2e-7, 4e-7], [0.0, 1.7e-7, 4e-7]),

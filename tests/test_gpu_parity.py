@@ -126,6 +126,34 @@ def test_pillar_workload_reduced_orders(gpu_ctx, nX):
             assert np.abs(res.trn[j, p] - TRN[p]).max() < TOL
 
 
+def test_eigen_recovery_on_gpu(gpu_ctx, monkeypatch):
+    """The reference's recovery of a failed eigen solve (rcwa/MKLEigenSolver.hpp:
+    81-93, 160-176), forced on every layer by a negative acceptance tolerance:
+    each patterned layer is solved again on A + diag(1e-11 (i+1) max|A|) and
+    reported with EIG_PERTURBED (and EIG_RESIDUAL, since the forced test fails
+    again).  The per-order results match the oracle taking the same branch
+    and stay within the parity bar of the unperturbed solve."""
+    from openrcwa_amd import _lib
+    from tests import gpu_helpers as gh
+    w, idx = pillar_subset(3, [0, 9, 1500])
+    base = gpu_pillar(gpu_ctx, w, idx)
+    assert not base.status.any()
+    gh.set_eig_tolerance(gpu_ctx, -1.0)
+    try:
+        res = gpu_pillar(gpu_ctx, w, idx)
+    finally:
+        gh.set_eig_tolerance(gpu_ctx, 1e-8)
+    assert (res.status == (_lib.STATUS_EIG_PERTURBED | _lib.STATUS_EIG_RESIDUAL)).all()
+    monkeypatch.setattr(o, "EIG_TOL", -1.0)
+    for j, i in enumerate(idx):
+        REF, TRN, _ = oracle_pillar(w, i)
+        for p in range(2):
+            assert np.abs(res.ref[j, p] - REF[p]).max() < TOL
+            assert np.abs(res.trn[j, p] - TRN[p]).max() < TOL
+    assert np.abs(res.ref - base.ref).max() < 1e-8 and np.abs(res.trn - base.trn).max() < 1e-8
+    assert np.abs(res.ref - base.ref).max() > 0.0
+
+
 def test_plans_reuse_cached_buffers(gpu_ctx):
     """Plans take their device buffers from the context's cache of destroyed
     plans' buffers, which still hold the old plan's data: plans of different
@@ -267,3 +295,25 @@ def test_uneven_stacks_against_oracle(gpu_ctx):
     for (ref, trn, _), (ref5, trn5, st5) in zip(res, res5):
         assert st5 == 0
         assert np.abs(ref5 - ref).max() < 1e-11 and np.abs(trn5 - trn).max() < 1e-11
+
+
+def test_uneven_growing_mode_is_flagged(gpu_ctx):
+    """Stack 326 of the 1024-stack uneven batch holds a Lorentz inclusion at
+    eps = -1.56 + 0.16i whose layer carries a mode growing by ~5e23 across its
+    282 nm (Im keff < 0 under the GammaSqrtOp branch): the reference's cascade
+    then loses all accuracy (the oracle's R moves by 1e-3 when a thickness
+    changes by 1e-12).  It is reported with ORCWA_STATUS_ILLCOND; a
+    well-conditioned stack of the same batch solved beside it is not, and
+    matches the oracle."""
+    from openrcwa_amd import _lib
+    allst = workloads.uneven_stacks(1024)
+    good = [i for i in range(300, 340) if allst[i]["nX"] == 6 and i != 326][:1]
+    stacks = [allst[326]] + [allst[i] for i in good]
+    res = engine.solve_uneven(gpu_ctx, stacks)
+    assert res[0][2] & _lib.STATUS_ILLCOND
+    for s_, (ref, trn, st) in zip(stacks[1:], res[1:]):
+        assert st == 0
+        REF, TRN, _ = _oracle_stack(s_)
+        for q in range(2):
+            assert np.abs(ref[q] - REF[q]).max() < 1e-9
+            assert np.abs(trn[q] - TRN[q]).max() < 1e-9

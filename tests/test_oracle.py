@@ -328,6 +328,40 @@ def test_eigen_backend_residual():
     assert st.solves == 1 and st.fallbacks == 0 and st.max_residual < 1e-12
 
 
+def test_eigen_recovery_branch_perturbation(monkeypatch):
+    # the recovery solve of rcwa/MKLEigenSolver.hpp:81-93, forced by a negative
+    # acceptance tolerance: on a diagonal matrix its eigenvalues are the
+    # diagonal shifted by exactly 1e-11 (i + 1) max|A|, and its residual is
+    # reported against the unperturbed matrix
+    monkeypatch.setattr(o, "EIG_TOL", -1.0)
+    diag = np.array([3.0 + 1j, -2.0, 0.5j, 7.0, 1.0 - 4j])
+    A = np.diag(diag)
+    st = o.EigenStats()
+    d, V = o.eig_backend(A, st)
+    assert st.fallbacks == 1
+    scale = np.abs(diag).max()
+    want = diag + 1e-11 * scale * np.arange(1, 6)
+    order = [int(np.argmin(np.abs(want - x))) for x in d]
+    assert sorted(order) == list(range(5))
+    assert np.abs(d - want[order]).max() < 1e-15 * scale
+    # residual against A, not A + D: the perturbation itself, 1e-11 scale i
+    assert abs(st.max_residual - 1e-11 * 5 * scale / scale) < 1e-13
+
+
+def test_eigen_recovery_keeps_patterned_results(monkeypatch):
+    # every layer of a patterned 2D stack through the recovery branch: R and T
+    # per order move by far less than the 1e-8 parity bar (the reference's
+    # comment puts the eigenvalue shift at ~1e-11 |A|)
+    layers = [L(1.0), L(6.0, 1.0, fill=0.4, d=1.5e-7, e1i=0.05, fy=0.6), L(2.25)]
+    a = o.ofd_solve_frequency(3, 4e-7, layers, 5.5e14, theta=15, phi=30)
+    monkeypatch.setattr(o, "EIG_TOL", -1.0)
+    b = o.ofd_solve_frequency(3, 4e-7, layers, 5.5e14, theta=15, phi=30)
+    for p in (0, 1):
+        assert np.abs(a["REF"][p] - b["REF"][p]).max() < 1e-8
+        assert np.abs(a["TRN"][p] - b["TRN"][p]).max() < 1e-8
+        assert abs(a["R"][p] - b["R"][p]) > 0.0 or abs(a["T"][p] - b["T"][p]) > 0.0
+
+
 def test_slab_scatter_matrix_uniform_closed_form():
     # a slab identical to the reference medium: no reflection, pure phase
     # (tests/test_core_integrators.cpp:79-112 checks the same closed form)

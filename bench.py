@@ -89,6 +89,8 @@ def bench_uneven(args):
     stacks = workloads.uneven_stacks(1024)
     ctx = engine.Context(0)
     print(f"[bench] workload uneven: {len(stacks)} stacks", file=sys.stderr, flush=True)
+    if args.graph:
+        return bench_uneven_graph(args, ctx, stacks)
     for _ in range(args.warmup):
         engine.solve_uneven(ctx, stacks, streams=args.streams, pad_layers=not args.no_pad_layers)
     t0 = time.perf_counter()
@@ -116,6 +118,166 @@ def bench_uneven(args):
     print(json.dumps(out), flush=True)
 
 
+def _uneven_flops(stacks):
+    import numpy as np
+    return sum(solve_flops((2 * s["nX"] + 1) * (2 * s["nY"] + 1),
+                           sum(1 for g in s["eps"][1:-1] if not np.all(g == g.flat[0])),
+                           len(s["thickness"])) for s in stacks)
+
+
+def bench_uneven_graph(args, ctx, stacks):
+    """The uneven batch with every group's plan resident and the executes of
+    all groups captured once in one HIP graph (forks on --graph-streams
+    streams): a step is one graph launch plus the results' copy to the host;
+    plan builds and the capture happen once, before the timed region."""
+    from openrcwa_amd import engine
+    t0 = time.perf_counter()
+    ub = engine.UnevenBatch(ctx, stacks, streams=args.graph_streams)
+    build_s = time.perf_counter() - t0
+    print(f"[bench] {len(ub.plans)} plans built and captured in {build_s:.1f} s", file=sys.stderr,
+          flush=True)
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        ub.run()
+        print(f"[bench] warmup {i}: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        t1 = time.perf_counter()
+        res = ub.run()
+        print(f"[bench] step {i}: {time.perf_counter() - t1:.2f} s", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    out = {"metric": "layer-stack solves/sec, uneven batch (N 3..12, 1..10 layers)",
+           "value": args.steps * len(stacks) / dt, "unit": "solves/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 (complex128)", "data": "synthetic (seeded splitmix64)",
+           "config": {"workload": f"uneven: {len(stacks)} stacks in {len(ub.plans)} resident "
+                                  f"plans, one captured HIP graph on {args.graph_streams} forks; "
+                                  f"a step is a graph launch plus the results' host copy",
+                      "batch": len(stacks)},
+           "plan_build_and_capture_s": build_s,
+           "eager_reruns": ub.graph.reruns,
+           "status_nonzero": sum(1 for r in res if r[2] != 0),
+           "alg_tflops_whole_solve": _uneven_flops(stacks) / (dt / args.steps) / 1e12}
+    ub.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_precision(args, ctx, w, ref_res, B, h, n_pat, fp64_s):
+    """The same batch with the star-product (cascade) GEMMs in each reduced
+    precision: solves/s and max |dR|, |dT| against the all-FP64 results."""
+    import numpy as np
+    from openrcwa_amd import engine
+    out = {}
+    precs = ["fp32", "bf16x3", "bf16x6"] if args.gemm_precision == "all" else \
+        [args.gemm_precision]
+    for prec in precs:
+        ctx.set_gemm_precision(prec)
+        plan = engine.Plan(ctx, w["nX"], w["nY"], w["coord_x"], w["coord_y"], w["thickness"],
+                           w["lam"], w["kx0"], w["ky0"], w["eps"], w["pols"])
+        ctx.set_gemm_precision("fp64")
+        plan.execute()
+        plan.sync()
+        t0 = time.perf_counter()
+        casc = 0.0
+        for _ in range(args.steps):
+            plan.execute()
+            plan.sync()
+            casc += plan.phase_ms(3)
+        dt = (time.perf_counter() - t0) / args.steps
+        r = plan.results()
+        plan.close()
+        d = max(float(np.abs(r.ref - ref_res.ref).max()), float(np.abs(r.trn - ref_res.trn).max()))
+        out[prec] = {"solves_per_s": B / dt, "speedup_vs_fp64": fp64_s / dt,
+                     "cascade_ms": casc / args.steps, "max_abs_dRdT_vs_fp64": d,
+                     "status_nonzero": int((r.status != 0).sum())}
+        print(f"[bench] {prec}: {dt:.2f} s per batch, max |dR,dT| vs fp64 {d:.2e}",
+              file=sys.stderr, flush=True)
+    return out
+
+
+def bench_surrogate(args):
+    """The learned surrogate of the two-layer 2D grating (Nx = Ny = 5, 12
+    parameters -> 18 per-order efficiencies): GPU data generation (2 rows per
+    solve, TE and TM), training of the 4-member 4 x 512 GELU ensemble with the
+    engine's HIP kernels (one graph-captured step replayed), calibration of the
+    error estimate, then 65536 held-out queries answered by the surrogate with
+    plain-solver fallback above 1e-3, against the plain GPU solver on all of
+    them.  value = queries/s of the surrogate path, fallback included."""
+    import numpy as np
+    from openrcwa_amd import engine, surrogate as sg
+    ctx = engine.Context(0)
+    n_solves = args.sur_solves
+    t0 = time.perf_counter()
+    def progress(done, total):
+        if done % (8 * 8192) == 0 or done == total:
+            print(f"[bench] data: {done} / {total} solves, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+    P, T, st = sg.make_dataset(ctx, n_solves, chunk=8192, progress=progress)
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] surrogate data: {n_solves} solves ({2 * n_solves} rows) in {gen_s:.1f} s",
+          file=sys.stderr, flush=True)
+    sur = sg.Surrogate(ctx, members=4, width=512, depth=4, seed=21)
+    t0 = time.perf_counter()
+    loss = sur.fit(P, T, epochs=args.sur_epochs, batch=4096, lr_max=2e-3, lr_min=1e-5)
+    train_s = time.perf_counter() - t0
+    train_gpu_ms = sur.net.last_ms()
+    print(f"[bench] training: {args.sur_epochs} epochs in {train_s:.1f} s, loss {loss[0]:.2e} -> "
+          f"{loss[-1]:.2e}", file=sys.stderr, flush=True)
+    cal = sg.sample_queries(16384, sg.SEED_CALIB)
+    cal_t, _ = sg.solve_queries(ctx, cal, chunk=8192)
+    kappa, cal_frac = sur.calibrate(cal, cal_t)
+    q = sg.sample_queries(65536, sg.SEED_HELDOUT)
+    sg.solve_queries(ctx, q[:512])                     # warm the plain path
+    t0 = time.perf_counter()
+    plain, pst = sg.solve_queries(ctx, q, chunk=8192)
+    plain_s = time.perf_counter() - t0
+    sur.query(q[:512])                                  # warm the surrogate path
+    ans, flagged, tm = sur.query(q)
+    err = float(np.abs(ans - plain).max())
+    mean, _ = sur.net.predict(sg.features(q))
+    net_err = np.abs(mean - plain).max(1)
+    sur.net.set_precision(True)
+    mean16, _ = sur.net.predict(sg.features(q))
+    bf16_ms = sur.net.last_ms()
+    sur.net.set_precision(False)
+    nq = q.shape[0]
+    flop = 2.0 * sur.net.P * sur.net.members * nq
+    value = nq / tm["total_s"]
+    plain_rate = nq / plain_s
+    out = {"metric": "surrogate queries/sec with plain-solver fallback at 1e-3 (2-layer 2D "
+                     "grating, 11x11 orders); max |dR,dT| vs plain GPU solver",
+           "value": value, "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": tm["total_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 network (FP32 MFMA), f64 fallback solves",
+           "data": "synthetic (seeded splitmix64); training data from the GPU solver",
+           "config": {"workload": f"surrogate: {2 * n_solves} training rows, 4 x (4 x 512 GELU) "
+                                  f"ensemble, {args.sur_epochs} epochs, {nq} held-out queries",
+                      "batch": nq},
+           "max_abs_dRdT_vs_plain_after_fallback": err,
+           "fallback_fraction": float(flagged.mean()),
+           "calibration": {"kappa": [float(k) for k in kappa], "fallback_fraction": cal_frac,
+                           "queries": 16384},
+           "network_only": {"queries_per_s": nq / tm["network_s"],
+                            "gpu_ms": tm["network_gpu_ms"],
+                            "tflops": flop / (tm["network_gpu_ms"] * 1e-3) / 1e12,
+                            "max_abs_err": float(net_err.max()),
+                            "median_abs_err": float(np.median(net_err)),
+                            "fraction_within_1e-3": float(np.mean(net_err <= 1e-3)),
+                            "bf16_gpu_ms": bf16_ms,
+                            "bf16_max_abs_diff_vs_fp32": float(np.abs(mean16 - mean).max())},
+           "plain_gpu": {"queries_per_s": plain_rate, "s": plain_s,
+                         "status_nonzero": int((pst != 0).sum())},
+           "speedup_vs_plain": value / plain_rate,
+           "data_generation": {"solves": n_solves, "s": gen_s, "solves_per_s": n_solves / gen_s},
+           "training": {"epochs": args.sur_epochs, "s": train_s, "gpu_ms": train_gpu_ms,
+                        "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
+                        "tflops": 6.0 * sur.net.P * sur.net.members * 2 * n_solves *
+                        args.sur_epochs / (train_gpu_ms * 1e-3) / 1e12}}
+    sur.net.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,10 +295,24 @@ def main():
     ap.add_argument("--no-pad-layers", action="store_true",
                     help="uneven workload: one batch per (order count, layer count) instead of "
                          "padding shorter stacks with zero-thickness exit-medium layers")
-    ap.add_argument("--workload", choices=["pillar2d", "large2d", "uneven"], default="pillar2d",
+    ap.add_argument("--sur-solves", type=int, default=1 << 19,
+                    help="surrogate workload: solves of training data (two rows each)")
+    ap.add_argument("--sur-epochs", type=int, default=100,
+                    help="surrogate workload: training epochs")
+    ap.add_argument("--workload", choices=["pillar2d", "large2d", "uneven", "surrogate"],
+                    default="pillar2d",
                     help="pillar2d: the metric's 15x15-order batch; large2d: 31x31 orders, "
                          "six patterned layers, 64 frequencies (no CPU baseline: one "
                          "oracle solve takes minutes)")
+    ap.add_argument("--graph", action="store_true",
+                    help="uneven workload: all groups resident, executed as one captured HIP graph")
+    ap.add_argument("--graph-streams", type=int, default=4,
+                    help="uneven workload with --graph: forks of the graph the groups run on")
+    ap.add_argument("--gemm-precision", default="fp64",
+                    choices=["fp64", "fp32", "bf16x3", "bf16x6", "all"],
+                    help="pillar2d / large2d: also run the batch with the star-product GEMMs "
+                         "in this reduced precision (all: each) and report its rate and its "
+                         "max |dR,dT| against the FP64 run (value stays the FP64 rate)")
     ap.add_argument("--no-pcie", action="store_true",
                     help="skip the host-to-host (plan + copies + execute) timing")
     args = ap.parse_args()
@@ -150,6 +326,8 @@ def main():
 
     if args.workload == "uneven":
         return bench_uneven(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
     large = args.workload == "large2d"
     if large:
         w = workloads.large2d(n_freq=args.n_freq or 64)
@@ -210,6 +388,11 @@ def main():
            "phase_ms": phases, "status_nonzero": int((res.status != 0).sum()),
            "alg_tflops_whole_solve": solve_flops(h, n_pat, n_pat + 2) * B / (dt / args.steps) / 1e12,
            "roofline": roof, "polarisation_solves_per_s": 2.0 * value}
+
+    if args.gemm_precision != "fp64":
+        plan.close()
+        out["mixed_precision"] = bench_precision(args, ctx, w, res, B, h, n_pat,
+                                                 dt / args.steps)
 
     if not args.no_pcie:
         # host buffers in, host buffers out: plan build + H2D + execute + D2H in one

@@ -139,6 +139,28 @@ int orcwa_gpu_destroy(orcwa_gpu_context *ctx);
  * reference solves one instance at a time). */
 int orcwa_gpu_set_workspace_limit(orcwa_gpu_context *ctx, size_t bytes);
 
+/* Precision of the star-product GEMMs: the two products per step of the
+ * RTCM cascade that form F + G = A + B R~ and F - G = B + A R~
+ * (rcwa/RCWASolver.cpp:433-443) in plans created later on this context, and
+ * the nine products of orcwa_gpu_redheffer_star and the integrators' star
+ * products (core/RCWAIntegrator.cpp:57-67).  The eigensolves, the interface
+ * (mode-matching) solves and the LU factorisations and solves of the cascade
+ * stay FP64 in every mode.  Reduced modes multiply on FP32 or BF16 MFMA and
+ * add each K step's partial sums in FP64:
+ *   ORCWA_GEMM_FP32    operands rounded to FP32 (24 significant bits)
+ *   ORCWA_GEMM_BF16X3  operands split into two BF16 slices (about 16 bits),
+ *                      three slice products
+ *   ORCWA_GEMM_BF16X6  three slices refined by the next slice's products (about
+ *                      24 bits), six slice products
+ * No reference counterpart (the reference multiplies in FP64). */
+enum orcwa_gemm_precision {
+  ORCWA_GEMM_FP64 = 0,
+  ORCWA_GEMM_FP32 = 1,
+  ORCWA_GEMM_BF16X3 = 2,
+  ORCWA_GEMM_BF16X6 = 3
+};
+int orcwa_gpu_set_gemm_precision(orcwa_gpu_context *ctx, int precision);
+
 /* One-shot batched solve: plan + execute + results + destroy.
  * Replaces the per-wavelength loop of sol/rcwa_bridge.cpp:244-334 and
  * rcwa/RCWADriver.cpp:282-368 (RCWASolver::solve + scatterPlaneWave). */
@@ -157,10 +179,33 @@ int orcwa_plan_sync(orcwa_plan *plan);
 int orcwa_plan_results(orcwa_plan *plan, orcwa_result *out);
 int orcwa_plan_destroy(orcwa_plan *plan);
 
+/* A HIP graph of the executes of several plans of one context, captured once
+ * and replayed: orcwa_graph_launch runs every plan's whole batch as one graph
+ * launch on the context stream (plan i on fork i mod n_streams of the graph,
+ * so plans of different order counts overlap on the device; each plan's
+ * chunks stay in order), orcwa_graph_sync waits, and orcwa_plan_results reads
+ * each plan's outputs as after an execute.  The captured form clears its
+ * status words by kernel and does no host synchronisation, so a failed eigen
+ * residual test is not recovered in the graph: orcwa_graph_sync then runs that
+ * plan again with orcwa_plan_execute (recovery included) before it returns.
+ * Plans used by a graph cannot be destroyed before the graph; a graph counts
+ * as a live plan of its context.  No reference counterpart: it replaces the
+ * per-wavelength loop of rcwa/RCWADriver.cpp:282 and sol/rcwa_bridge.cpp:244
+ * for batches of several stack layouts. */
+typedef struct orcwa_graph orcwa_graph;
+int orcwa_graph_create(orcwa_gpu_context *ctx, orcwa_plan *const *plans, int nplans,
+                       int n_streams, orcwa_graph **graph);
+int orcwa_graph_launch(orcwa_graph *graph);
+int orcwa_graph_sync(orcwa_graph *graph);
+/* plans re-run eagerly by the last orcwa_graph_sync */
+int orcwa_graph_reruns(const orcwa_graph *graph);
+int orcwa_graph_destroy(orcwa_graph *graph);
+
 /* Elapsed GPU time (ms) of named phases of the last execute, measured with HIP
  * events on the plan's stream.  phase: 0 total, 1 convolution matrices,
  * 2 P/Q + eigensolve + V, 3 interfaces + cascade, 4 plane-wave R/T,
- * 5 eigensolver only.  Returns a negative value if unknown. */
+ * 5 eigensolver only.  Returns a negative value if unknown (also after a graph
+ * launch, which records no events). */
 double orcwa_plan_phase_ms(orcwa_plan *plan, int phase);
 
 /* Slab scattering matrices in a reference basis, batched: for each instance b,

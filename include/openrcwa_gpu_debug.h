@@ -18,7 +18,8 @@ extern "C" {
 
 /* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; op = transpose (t) and/or
  * conjugate (c) flags per operand; A is M x K after op, B is K x N after op.
- * ldA/ldB/ldC are the column-major leading dimensions of the stored arrays. */
+ * ldA/ldB/ldC are the column-major leading dimensions of the stored arrays.
+ * The products run at the context's orcwa_gpu_set_gemm_precision setting. */
 int orcwa_dbg_zgemm(orcwa_gpu_context *ctx, int M, int N, int K, const double *alpha,
                     const double *A, int ldA, int tA, int cA, const double *B, int ldB, int tB,
                     int cB, const double *beta, double *C, int ldC, int batch);

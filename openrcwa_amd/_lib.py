@@ -23,6 +23,9 @@ STATUS_NOCONV = 8
 STATUS_EIG_PERTURBED = 16
 STATUS_ILLCOND = 32
 
+# orcwa_gemm_precision
+GEMM_PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
+
 
 class OrcwaStack(ctypes.Structure):
     _fields_ = [("nx_harm", c_int), ("ny_harm", c_int), ("n_cx", c_int), ("n_cy", c_int),
@@ -75,11 +78,31 @@ SLICES = {"xy": 0, "xz": 1, "yz": 2}
 
 # every symbol include/openrcwa_gpu.h declares
 EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_set_workspace_limit",
+            "orcwa_gpu_set_gemm_precision",
             "orcwa_gpu_solve_batch",
             "orcwa_plan_create", "orcwa_plan_execute", "orcwa_plan_sync",
             "orcwa_plan_results", "orcwa_plan_destroy", "orcwa_plan_phase_ms",
             "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "orcwa_gpu_field_image",
-            "orcwa_gpu_rcwa_integrate", "orcwa_gpu_redheffer_integrate", "rcwa_run"]
+            "orcwa_gpu_rcwa_integrate", "orcwa_gpu_redheffer_integrate", "rcwa_run",
+            "orcwa_graph_create", "orcwa_graph_launch", "orcwa_graph_sync", "orcwa_graph_reruns",
+            "orcwa_graph_destroy"]
+
+# every symbol include/openrcwa_surrogate.h declares
+EXPORTED_SURROGATE = ["orcwa_mlp_create", "orcwa_mlp_destroy", "orcwa_mlp_param_count",
+                      "orcwa_mlp_get_params", "orcwa_mlp_set_params", "orcwa_mlp_set_data",
+                      "orcwa_mlp_train", "orcwa_mlp_gradient", "orcwa_mlp_set_precision",
+                      "orcwa_mlp_set_error_scale",
+                      "orcwa_mlp_forward", "orcwa_mlp_predict", "orcwa_mlp_predict_device",
+                      "orcwa_mlp_last_ms"]
+
+MLP_FP32 = 0
+MLP_BF16 = 1
+
+
+class OrcwaMlpTrainOpts(ctypes.Structure):
+    _fields_ = [("epochs", c_int), ("batch", c_int), ("lr_max", ctypes.c_float),
+                ("lr_min", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float), ("seed", ctypes.c_ulonglong), ("use_graph", c_int)]
 
 _lib = None
 
@@ -100,6 +123,7 @@ def load(path: str | None = None):
     lib.orcwa_gpu_create.argtypes = [POINTER(c_void_p), c_int]
     lib.orcwa_gpu_destroy.argtypes = [c_void_p]
     lib.orcwa_gpu_set_workspace_limit.argtypes = [c_void_p, ctypes.c_size_t]
+    lib.orcwa_gpu_set_gemm_precision.argtypes = [c_void_p, c_int]
     lib.orcwa_gpu_solve_batch.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
                                           c_int, POINTER(OrcwaResult)]
     lib.orcwa_plan_create.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
@@ -110,6 +134,12 @@ def load(path: str | None = None):
     lib.orcwa_plan_destroy.argtypes = [c_void_p]
     lib.orcwa_plan_phase_ms.argtypes = [c_void_p, c_int]
     lib.orcwa_plan_phase_ms.restype = c_double
+    lib.orcwa_graph_create.argtypes = [c_void_p, POINTER(c_void_p), c_int, c_int,
+                                       POINTER(c_void_p)]
+    lib.orcwa_graph_launch.argtypes = [c_void_p]
+    lib.orcwa_graph_sync.argtypes = [c_void_p]
+    lib.orcwa_graph_reruns.argtypes = [c_void_p]
+    lib.orcwa_graph_destroy.argtypes = [c_void_p]
     lib.orcwa_gpu_rcwa_integrate.argtypes = [c_void_p, POINTER(OrcwaZStack),
                                              POINTER(OrcwaZResult)]
     lib.orcwa_gpu_redheffer_integrate.argtypes = [c_void_p, POINTER(OrcwaZSampler), c_double,
@@ -117,6 +147,25 @@ def load(path: str | None = None):
     lib.orcwa_gpu_field_image.argtypes = [c_void_p, POINTER(OrcwaStack), POINTER(OrcwaInstance),
                                           POINTER(OrcwaFieldRequest), POINTER(c_int),
                                           POINTER(c_int), POINTER(c_double), POINTER(c_int)]
+    c_fp = POINTER(ctypes.c_float)
+    lib.orcwa_mlp_create.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int,
+                                     ctypes.c_ulonglong, POINTER(c_void_p)]
+    lib.orcwa_mlp_destroy.argtypes = [c_void_p]
+    lib.orcwa_mlp_param_count.argtypes = [c_void_p]
+    lib.orcwa_mlp_param_count.restype = ctypes.c_longlong
+    lib.orcwa_mlp_get_params.argtypes = [c_void_p, c_fp]
+    lib.orcwa_mlp_set_params.argtypes = [c_void_p, c_fp]
+    lib.orcwa_mlp_set_data.argtypes = [c_void_p, c_fp, c_fp, ctypes.c_longlong]
+    lib.orcwa_mlp_train.argtypes = [c_void_p, POINTER(OrcwaMlpTrainOpts), c_fp]
+    lib.orcwa_mlp_gradient.argtypes = [c_void_p, c_fp, c_fp, c_int, c_fp, c_fp]
+    lib.orcwa_mlp_set_precision.argtypes = [c_void_p, c_int]
+    lib.orcwa_mlp_set_error_scale.argtypes = [c_void_p, c_fp]
+    lib.orcwa_mlp_forward.argtypes = [c_void_p, c_fp, ctypes.c_longlong, c_fp]
+    lib.orcwa_mlp_predict.argtypes = [c_void_p, c_fp, ctypes.c_longlong, c_fp, c_fp]
+    lib.orcwa_mlp_predict_device.argtypes = [c_void_p, c_void_p, ctypes.c_longlong, c_void_p,
+                                             c_void_p]
+    lib.orcwa_mlp_last_ms.argtypes = [c_void_p]
+    lib.orcwa_mlp_last_ms.restype = c_double
     _lib = lib
     return lib
 

@@ -24,7 +24,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 # .hip files hold device code; .cpp files are host-only C++ (no HIP headers)
 SOURCES = ["blas.hip", "eig.hip", "rcwa_kernels.hip", "field_kernels.hip", "plan.hip",
-           "smatrix.hip", "debug.hip", "ofd_bridge.cpp"]
+           "smatrix.hip", "mlp.hip", "debug.hip", "ofd_bridge.cpp"]
 
 
 def _run(cmd, verbose):
@@ -54,7 +54,25 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    build_cpp_harness(verbose)
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+HARNESS_SRC = os.path.join(ROOT, "tests", "cpp_facade_harness.cpp")
+HARNESS_BIN = os.path.join(ROOT, "tests", "_bin", "cpp_facade_harness")
+
+
+def build_cpp_harness(verbose: bool = False) -> str:
+    """A C++ caller of include/openrcwa_gpu.hpp linked against the library
+    (tests/cpp_facade_harness.cpp; the GPU tests run it)."""
+    if not os.path.exists(HARNESS_SRC):
+        return ""
+    os.makedirs(os.path.dirname(HARNESS_BIN), exist_ok=True)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), HARNESS_SRC,
+          "-L" + HERE, "-lopenrcwa_gpu", "-Wl,-rpath,$ORIGIN/../../openrcwa_amd",
+          "-o", HARNESS_BIN], verbose)
+    return HARNESS_BIN
 
 
 if __name__ == "__main__":

@@ -46,27 +46,16 @@ struct TileShape {
   static_assert(BI >= 1 && BJ >= 1 && WR * WC == 4, "tile shape");
 };
 
-template <bool BETA0, int TM, int TN>
-__global__ __launch_bounds__(256, 2) void k_zgemm(
-    int M, int N, int K, cd alpha, cd beta,
-    const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
-    const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
-    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
-  using TS = TileShape<TM, TN>;
-  constexpr int BI = TS::BI, BJ = TS::BJ, LA = TS::LA, LB = TS::LB, WC = TS::WC;
-  // rows padded by one double: the k-fast stores (16 lanes down one column)
-  // then fall in distinct banks
-  __shared__ double sAr[GEMM_KS][TM + GEMM_PAD];
-  __shared__ double sAi[GEMM_KS][TM + GEMM_PAD];
-  __shared__ double sBr[GEMM_KS][TN + GEMM_PAD];
-  __shared__ double sBi[GEMM_KS][TN + GEMM_PAD];
-
-  int bx = blockIdx.x, by = blockIdx.y, b = blockIdx.z;
+// Tile and batch entry of this workgroup.  Workgroups go to the 8 XCDs
+// round-robin by linear id: for large batches give each XCD whole matrices so
+// one L2 serves all tiles of a product (a few big products are better spread
+// over every XCD).
+__device__ __forceinline__ void tile_of_block(int& bx, int& by, int& b) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  b = blockIdx.z;
 #if ORCWA_GEMM_XCD
   if (gridDim.z >= ORCWA_GEMM_XCD_MIN) {
-    // workgroups go to the 8 XCDs round-robin by linear id: give each XCD
-    // whole matrices so one L2 serves all tiles of a product (large batches
-    // only: a few big products are better spread over every XCD)
     const unsigned T = gridDim.x * gridDim.y, Bt = gridDim.z;
     const unsigned w = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const unsigned full = (Bt / 8) * 8 * T;
@@ -84,6 +73,25 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
     by = (int)(tile / gridDim.x);
   }
 #endif
+}
+
+template <bool BETA0, int TM, int TN>
+__global__ __launch_bounds__(256, 2) void k_zgemm(
+    int M, int N, int K, cd alpha, cd beta,
+    const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
+    const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
+    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
+  using TS = TileShape<TM, TN>;
+  constexpr int BI = TS::BI, BJ = TS::BJ, LA = TS::LA, LB = TS::LB, WC = TS::WC;
+  // rows padded by one double: the k-fast stores (16 lanes down one column)
+  // then fall in distinct banks
+  __shared__ double sAr[GEMM_KS][TM + GEMM_PAD];
+  __shared__ double sAi[GEMM_KS][TM + GEMM_PAD];
+  __shared__ double sBr[GEMM_KS][TN + GEMM_PAD];
+  __shared__ double sBi[GEMM_KS][TN + GEMM_PAD];
+
+  int bx, by, b;
+  tile_of_block(bx, by, b);
   const cd* __restrict__ A = Ap[b] + offA;
   const cd* __restrict__ B = Bp[b] + offB;
   cd* __restrict__ C = Cp[b] + offC;
@@ -207,20 +215,348 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
       }
 }
 
+// ---------------------------------------------------------------------------
+// Reduced-precision complex GEMMs for the cascade (star-product) products,
+// same operands, tiles and XCD mapping as k_zgemm.  Products run on FP32 or
+// BF16 MFMA with FP32 accumulation inside one K step; each K step's partial
+// sums are added into FP64 accumulators, so only the operand rounding and
+// the short in-step sums are reduced-precision.
+//   FP32:    operands rounded to FP32, MFMA 16x16x4 f32, Gauss three-product
+//            complex form (as k_zgemm).
+//   BF16xS:  each FP64 operand split into S BF16 slices x = x0 + x1 + ...
+//            (x0 = bf16(x), x1 = bf16(x - x0), ...; 8 significant bits each),
+//            the slice products with i + j < S kept (S = 2: 3 products, about
+//            16 significant bits; S = 3: 6 products, about 24 -- the second
+//            level refines the first by the products of the next slice),
+//            MFMA 16x16x32 bf16, four-product complex form.
+// MFMA f32 16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[i][j] in lane j + 16 (i / 4), element i % 4.  MFMA 16x16x32 bf16:
+// A[i][k] in lane i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8),
+// element k % 8; D as f32.
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+template <bool BETA0, int TM, int TN>
+__global__ __launch_bounds__(256, 2) void k_zgemm_f32(
+    int M, int N, int K, cd alpha, cd beta,
+    const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
+    const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
+    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
+  constexpr int KS = 16;
+  constexpr int WR = (TM >= 32 && TN >= 32) ? 2 : (TN == 16 ? 4 : 1), WC = 4 / WR;
+  constexpr int BI = TM / (16 * WR), BJ = TN / (16 * WC);
+  constexpr int LA = TM * KS / 256, LB = TN * KS / 256;
+  __shared__ float sAr[KS][TM + 1], sAi[KS][TM + 1];
+  __shared__ float sBr[KS][TN + 1], sBi[KS][TN + 1];
+  int bx, by, b;
+  tile_of_block(bx, by, b);
+  const cd* __restrict__ A = Ap[b] + offA;
+  const cd* __restrict__ B = Bp[b] + offB;
+  cd* __restrict__ C = Cp[b] + offC;
+  const int m0 = bx * TM, n0 = by * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave / WC) * (16 * BI), wn = (wave % WC) * (16 * BJ);
+  const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
+  const bool a_mfast = (rsA == 1), b_kfast = (rsB == 1);
+  int amm[LA], akk[LA], bkk[LB], bnn[LB];
+  const cd* pa[LA];
+  const cd* pb[LB];
+  bool aok[LA], bok[LB];
+#pragma unroll
+  for (int j = 0; j < LA; ++j) {
+    const int e = tid + 256 * j;
+    amm[j] = a_mfast ? (e % TM) : (e / KS);
+    akk[j] = a_mfast ? (e / TM) : (e % KS);
+    aok[j] = (m0 + amm[j] < M);
+    pa[j] = A + (long long)(m0 + amm[j]) * rsA + (long long)akk[j] * csA;
+  }
+#pragma unroll
+  for (int j = 0; j < LB; ++j) {
+    const int e = tid + 256 * j;
+    bkk[j] = b_kfast ? (e % KS) : (e / TN);
+    bnn[j] = b_kfast ? (e / KS) : (e % TN);
+    bok[j] = (n0 + bnn[j] < N);
+    pb[j] = B + (long long)bkk[j] * rsB + (long long)(n0 + bnn[j]) * csB;
+  }
+  const long long stepA = KS * csA, stepB = KS * rsB;
+  cd ra[LA], rb[LB];
+#pragma unroll
+  for (int j = 0; j < LA; ++j) ra[j] = (aok[j] && akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
+#pragma unroll
+  for (int j = 0; j < LB; ++j) rb[j] = (bok[j] && bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
+
+  double d1[BI][BJ][4], d2[BI][BJ][4], d3[BI][BJ][4];
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) d1[i][j][r] = d2[i][j][r] = d3[i][j][r] = 0.0;
+
+  for (int k0 = 0; k0 < K; k0 += KS) {
+#pragma unroll
+    for (int j = 0; j < LA; ++j) {
+      sAr[akk[j]][amm[j]] = (float)ra[j].re;
+      sAi[akk[j]][amm[j]] = (float)(sa * ra[j].im);
+    }
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      sBr[bkk[j]][bnn[j]] = (float)rb[j].re;
+      sBi[bkk[j]][bnn[j]] = (float)(sb * rb[j].im);
+    }
+    __syncthreads();
+    if (k0 + KS < K) {
+#pragma unroll
+      for (int j = 0; j < LA; ++j) {
+        pa[j] += stepA;
+        ra[j] = (aok[j] && k0 + KS + akk[j] < K) ? pa[j][0] : mk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        pb[j] += stepB;
+        rb[j] = (bok[j] && k0 + KS + bkk[j] < K) ? pb[j][0] : mk(0.0, 0.0);
+      }
+    }
+    v4f p1[BI][BJ], p2[BI][BJ], p3[BI][BJ];
+#pragma unroll
+    for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+        p1[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        p2[i][j] = p1[i][j];
+        p3[i][j] = p1[i][j];
+      }
+#pragma unroll
+    for (int kq = 0; kq < KS; kq += 4) {
+      const int ka = kq + (lane >> 4);
+      float ar[BI], ai[BI], as[BI], br[BJ], bi[BJ], bs[BJ];
+#pragma unroll
+      for (int i = 0; i < BI; ++i) {
+        ar[i] = sAr[ka][wm + 16 * i + (lane & 15)];
+        ai[i] = sAi[ka][wm + 16 * i + (lane & 15)];
+        as[i] = ar[i] + ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+        br[j] = sBr[ka][wn + 16 * j + (lane & 15)];
+        bi[j] = sBi[ka][wn + 16 * j + (lane & 15)];
+        bs[j] = br[j] + bi[j];
+      }
+#pragma unroll
+      for (int i = 0; i < BI; ++i)
+#pragma unroll
+        for (int j = 0; j < BJ; ++j) {
+          p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], p1[i][j], 0, 0, 0);
+          p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], bi[j], p2[i][j], 0, 0, 0);
+          p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[i], bs[j], p3[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          d1[i][j][r] += (double)p1[i][j][r];
+          d2[i][j][r] += (double)p2[i][j][r];
+          d3[i][j][r] += (double)p3[i][j][r];
+        }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int gn = n0 + wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N) {
+          const double t1 = d1[i][j][r], t2 = d2[i][j][r];
+          cd v = alpha * mk(t1 - t2, d3[i][j][r] - t1 - t2);
+          cd* p = C + gm * rsC + gn * csC;
+          if (!BETA0) v += beta * (*p);
+          *p = v;
+        }
+      }
+}
+
+// x -> S BF16 slices (round to nearest at each level)
+template <int S>
+__device__ __forceinline__ void bf16_split(double x, __bf16 (&out)[S]) {
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const __bf16 h = (__bf16)(float)x;
+    out[q] = h;
+    x -= (double)(float)h;
+  }
+}
+
+__device__ __forceinline__ v8bf bf16_neg(v8bf x) {
+  v8s t = __builtin_bit_cast(v8s, x);
+  t ^= (v8s)((short)0x8000);
+  return __builtin_bit_cast(v8bf, t);
+}
+
+template <bool BETA0, int TM, int TN, int S>
+__global__ __launch_bounds__(256, 2) void k_zgemm_bf16(
+    int M, int N, int K, cd alpha, cd beta,
+    const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
+    const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
+    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
+  constexpr int KS = 32, LDK = KS + 8;        // k-fastest rows, padded by 8 (16 bytes)
+  constexpr int WR = (TM >= 32 && TN >= 32) ? 2 : (TN == 16 ? 4 : 1), WC = 4 / WR;
+  constexpr int BI = TM / (16 * WR), BJ = TN / (16 * WC);
+  constexpr int LA = TM * KS / 256, LB = TN * KS / 256;
+  // [re/im][slice][row][k]
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][S][TM][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][S][TN][LDK];
+  int bx, by, b;
+  tile_of_block(bx, by, b);
+  const cd* __restrict__ A = Ap[b] + offA;
+  const cd* __restrict__ B = Bp[b] + offB;
+  cd* __restrict__ C = Cp[b] + offC;
+  const int m0 = bx * TM, n0 = by * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave / WC) * (16 * BI), wn = (wave % WC) * (16 * BJ);
+  const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
+  const bool a_mfast = (rsA == 1), b_kfast = (rsB == 1);
+  // element j of this thread's share of a tile: (row, k) of A and (k, col) of
+  // B, recomputed from the thread index (fewer live registers than tables)
+  auto a_at = [&](int j, int& mm, int& kk) {
+    const int e = tid + 256 * j;
+    mm = a_mfast ? (e % TM) : (e / KS);
+    kk = a_mfast ? (e / TM) : (e % KS);
+  };
+  auto b_at = [&](int j, int& kk, int& nn) {
+    const int e = tid + 256 * j;
+    kk = b_kfast ? (e % KS) : (e / TN);
+    nn = b_kfast ? (e / KS) : (e % TN);
+  };
+
+  double dre[BI][BJ][4], dim_[BI][BJ][4];
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dre[i][j][r] = dim_[i][j][r] = 0.0;
+
+  for (int k0 = 0; k0 < K; k0 += KS) {
+    // no register prefetch (the slices need the registers); the second
+    // workgroup on the CU covers the load latency
+#pragma unroll
+    for (int j = 0; j < LA; ++j) {
+      int mm, kk;
+      a_at(j, mm, kk);
+      const cd v = (m0 + mm < M && k0 + kk < K)
+                       ? A[(long long)(m0 + mm) * rsA + (long long)(k0 + kk) * csA]
+                       : mk(0.0, 0.0);
+      __bf16 xr[S], xi[S];
+      bf16_split<S>(v.re, xr);
+      bf16_split<S>(sa * v.im, xi);
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        sA[0][q][mm][kk] = xr[q];
+        sA[1][q][mm][kk] = xi[q];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      int kk, nn;
+      b_at(j, kk, nn);
+      const cd v = (n0 + nn < N && k0 + kk < K)
+                       ? B[(long long)(k0 + kk) * rsB + (long long)(n0 + nn) * csB]
+                       : mk(0.0, 0.0);
+      __bf16 xr[S], xi[S];
+      bf16_split<S>(v.re, xr);
+      bf16_split<S>(sb * v.im, xi);
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        sB[0][q][nn][kk] = xr[q];
+        sB[1][q][nn][kk] = xi[q];
+      }
+    }
+    __syncthreads();
+    const int kl = 8 * (lane >> 4), rl = lane & 15;
+#pragma unroll
+    for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+        v4f pre = (v4f){0.f, 0.f, 0.f, 0.f}, pim = pre;
+#pragma unroll
+        for (int qa = 0; qa < S; ++qa) {
+          const v8bf a_r = *(const v8bf*)&sA[0][qa][wm + 16 * i + rl][kl];
+          const v8bf a_i = *(const v8bf*)&sA[1][qa][wm + 16 * i + rl][kl];
+          const v8bf a_in = bf16_neg(a_i);
+#pragma unroll
+          for (int qb = 0; qb + qa < S; ++qb) {
+            const v8bf b_r = *(const v8bf*)&sB[0][qb][wn + 16 * j + rl][kl];
+            const v8bf b_i = *(const v8bf*)&sB[1][qb][wn + 16 * j + rl][kl];
+            // re += ar br - ai bi, im += ar bi + ai br
+            pre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_r, b_r, pre, 0, 0, 0);
+            pre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_in, b_i, pre, 0, 0, 0);
+            pim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_r, b_i, pim, 0, 0, 0);
+            pim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_i, b_r, pim, 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          dre[i][j][r] += (double)pre[r];
+          dim_[i][j][r] += (double)pim[r];
+        }
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int gn = n0 + wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N) {
+          cd v = alpha * mk(dre[i][j][r], dim_[i][j][r]);
+          cd* p = C + gm * rsC + gn * csC;
+          if (!BETA0) v += beta * (*p);
+          *p = v;
+        }
+      }
+}
+
 template <int TM, int TN>
 static void zgemm_launch(hipStream_t s, int M, int N, int K, cd alpha, const cd* const* A,
                          long long offA, long long rsA, long long csA, int conjA,
                          const cd* const* B, long long offB, long long rsB, long long csB,
                          int conjB, cd beta, cd* const* C, long long offC, long long rsC,
-                         long long csC, int batch) {
+                         long long csC, int batch, int prec) {
   if (M <= 0 || N <= 0) return;
   dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, batch);
-  if (beta.re == 0.0 && beta.im == 0.0)
-    hipLaunchKernelGGL((k_zgemm<true, TM, TN>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
-                       offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
-  else
-    hipLaunchKernelGGL((k_zgemm<false, TM, TN>), grid, dim3(256), 0, s, M, N, K, alpha, beta, A,
-                       offA, rsA, csA, conjA, B, offB, rsB, csB, conjB, C, offC, rsC, csC);
+  const bool b0 = (beta.re == 0.0 && beta.im == 0.0);
+#define ORCWA_GEMM_ARGS                                                                        \
+  grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA, rsA, csA, conjA, B, offB, rsB, csB,   \
+      conjB, C, offC, rsC, csC
+  switch (prec) {
+    case GEMM_FP32:
+      if (b0) hipLaunchKernelGGL((k_zgemm_f32<true, TM, TN>), ORCWA_GEMM_ARGS);
+      else hipLaunchKernelGGL((k_zgemm_f32<false, TM, TN>), ORCWA_GEMM_ARGS);
+      break;
+    case GEMM_BF16X3:
+      if (b0) hipLaunchKernelGGL((k_zgemm_bf16<true, TM, TN, 2>), ORCWA_GEMM_ARGS);
+      else hipLaunchKernelGGL((k_zgemm_bf16<false, TM, TN, 2>), ORCWA_GEMM_ARGS);
+      break;
+    case GEMM_BF16X6:
+      if (b0) hipLaunchKernelGGL((k_zgemm_bf16<true, TM, TN, 3>), ORCWA_GEMM_ARGS);
+      else hipLaunchKernelGGL((k_zgemm_bf16<false, TM, TN, 3>), ORCWA_GEMM_ARGS);
+      break;
+    default:
+      if (b0) hipLaunchKernelGGL((k_zgemm<true, TM, TN>), ORCWA_GEMM_ARGS);
+      else hipLaunchKernelGGL((k_zgemm<false, TM, TN>), ORCWA_GEMM_ARGS);
+  }
+#undef ORCWA_GEMM_ARGS
 }
 
 // rows of a strip narrower than 64: the smallest tile height that covers them
@@ -228,31 +564,32 @@ template <int TN>
 static void zgemm_rows(hipStream_t s, int M, int N, int K, cd alpha, const cd* const* A,
                        long long offA, long long rsA, long long csA, int conjA, const cd* const* B,
                        long long offB, long long rsB, long long csB, int conjB, cd beta,
-                       cd* const* C, long long offC, long long rsC, long long csC, int batch) {
+                       cd* const* C, long long offC, long long rsC, long long csC, int batch,
+                       int prec) {
   // (a 4-wave tile needs at least four 16 x 16 blocks)
   if constexpr (TN == 64) {
     if (M <= 16) {
       zgemm_launch<16, 64>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                           beta, C, offC, rsC, csC, batch);
+                           beta, C, offC, rsC, csC, batch, prec);
       return;
     }
   }
   if constexpr (TN >= 32) {
     if (M <= 32) {
       zgemm_launch<32, TN>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                           beta, C, offC, rsC, csC, batch);
+                           beta, C, offC, rsC, csC, batch, prec);
       return;
     }
   }
   zgemm_launch<64, TN>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                       beta, C, offC, rsC, csC, batch);
+                       beta, C, offC, rsC, csC, batch, prec);
 }
 
-void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
-                   const cd* const* A, long long offA, long long rsA, long long csA, int conjA,
-                   const cd* const* B, long long offB, long long rsB, long long csB, int conjB,
-                   cd beta, cd* const* C, long long offC, long long rsC, long long csC,
-                   int batch) {
+void zgemm_batched_prec(int prec, hipStream_t s, int M, int N, int K, cd alpha,
+                        const cd* const* A, long long offA, long long rsA, long long csA,
+                        int conjA, const cd* const* B, long long offB, long long rsB,
+                        long long csB, int conjB, cd beta, cd* const* C, long long offC,
+                        long long rsC, long long csC, int batch) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   // 64 x 64 tiles over the largest multiple-of-64 block, narrower tiles on the
   // remaining row and column strips (n = 450: 448 + a 2-wide strip, instead
@@ -260,22 +597,31 @@ void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
   const int M0 = (M / 64) * 64, N0 = (N / 64) * 64, mr = M - M0, nr = N - N0;
   if (M0 > 0 && N0 > 0)
     zgemm_launch<64, 64>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                         beta, C, offC, rsC, csC, batch);
+                         beta, C, offC, rsC, csC, batch, prec);
   if (mr > 0 && N0 > 0)     // bottom strip
     zgemm_rows<64>(s, mr, N0, K, alpha, A, offA + M0 * rsA, rsA, csA, conjA, B, offB, rsB, csB,
-                   conjB, beta, C, offC + M0 * rsC, rsC, csC, batch);
+                   conjB, beta, C, offC + M0 * rsC, rsC, csC, batch, prec);
   if (nr > 0) {             // right strip, all rows (corner included)
     const long long oB = offB + N0 * csB, oC = offC + N0 * csC;
     if (nr <= 16)
       zgemm_rows<16>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
-                     oC, rsC, csC, batch);
+                     oC, rsC, csC, batch, prec);
     else if (nr <= 32)
       zgemm_rows<32>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
-                     oC, rsC, csC, batch);
+                     oC, rsC, csC, batch, prec);
     else
       zgemm_rows<64>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
-                     oC, rsC, csC, batch);
+                     oC, rsC, csC, batch, prec);
   }
+}
+
+void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
+                   const cd* const* A, long long offA, long long rsA, long long csA, int conjA,
+                   const cd* const* B, long long offB, long long rsB, long long csB, int conjB,
+                   cd beta, cd* const* C, long long offC, long long rsC, long long csC,
+                   int batch) {
+  zgemm_batched_prec(GEMM_FP64, s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
+                     conjB, beta, C, offC, rsC, csC, batch);
 }
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,9 @@ struct orcwa_gpu_context {
   // residual acceptance of the eigen solves (orcwa_dbg_set_eig_tolerance;
   // 1e-8 as rcwa/MKLEigenSolver.hpp:75-79)
   double eig_tol = 1e-8;
+  // precision of the cascade / star-product GEMMs of later plans and star
+  // products (orcwa_gpu_set_gemm_precision; GemmPrec values)
+  int gemm_prec = 0;
 };
 
 namespace orcwa {

@@ -64,9 +64,9 @@ int orcwa_dbg_zgemm(orcwa_gpu_context* ctx, int M, int N, int K, const double* a
   // op(A)[i,k] = A[i*rs + k*cs]
   const long long rsA = tA ? ldA : 1, csA = tA ? 1 : ldA;
   const long long rsB = tB ? ldB : 1, csB = tB ? 1 : ldB;
-  zgemm_batched(s, M, N, K, mk(alpha[0], alpha[1]), (const cd* const*)a.table, 0, rsA, csA, cA,
-                (const cd* const*)b.table, 0, rsB, csB, cB, mk(beta[0], beta[1]), c.table, 0, 1,
-                ldC, batch);
+  zgemm_batched_prec(ctx->gemm_prec, s, M, N, K, mk(alpha[0], alpha[1]),
+                     (const cd* const*)a.table, 0, rsA, csA, cA, (const cd* const*)b.table, 0,
+                     rsB, csB, cB, mk(beta[0], beta[1]), c.table, 0, 1, ldC, batch);
   DCHK(hipGetLastError());
   DCHK(hipMemcpyAsync(C, c.data, sizeof(cd) * eC * batch, hipMemcpyDeviceToHost, s));
   DCHK(hipStreamSynchronize(s));

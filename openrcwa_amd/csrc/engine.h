@@ -25,6 +25,17 @@ void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
                    cd beta, cd* const* C, long long offC, long long rsC, long long csC,
                    int batch);
 
+// Precision of the products of zgemm_batched_prec (the cascade / star-product
+// GEMMs); the values are those of orcwa_gemm_precision in openrcwa_gpu.h.
+enum GemmPrec { GEMM_FP64 = 0, GEMM_FP32 = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3 };
+// zgemm_batched with the products in FP64 (GEMM_FP64: the same kernels), FP32
+// or split BF16 on MFMA, every K step's partial sums accumulated in FP64.
+void zgemm_batched_prec(int prec, hipStream_t s, int M, int N, int K, cd alpha,
+                        const cd* const* A, long long offA, long long rsA, long long csA,
+                        int conjA, const cd* const* B, long long offB, long long rsB,
+                        long long csB, int conjB, cd beta, cd* const* C, long long offC,
+                        long long rsC, long long csC, int batch);
+
 // Batched LU with partial pivoting of n x n column-major matrices (lda = n),
 // in place.  ipiv[b*n + i] = global row swapped with row i.  info[b] gets
 // ORCWA_STATUS_SINGULAR or-ed in when an exactly zero pivot is met

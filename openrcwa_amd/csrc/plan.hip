@@ -145,6 +145,7 @@ struct orcwa_plan {
   // field plans (one instance): per-step interface matrices, the Tdd column
   // chain at the top of every layer, and the instance's layer data
   bool keep = false;
+  int gemm_prec = 0;             // GemmPrec of the cascade products (from the context)
   // modes-only plans stop after the layer modes (no cascade, no R/T); their
   // layers may each have their own cell edges (one edge set per grid)
   bool modes_only = false;
@@ -165,6 +166,7 @@ struct orcwa_plan {
 
   hipEvent_t ev[8] = {};
   bool have_events = false;
+  int live_graphs = 0;           // orcwa_graph objects that captured this plan
   double phase_ms[6] = {-1, -1, -1, -1, -1, -1};
   double eig_ms_acc = 0.0;
 
@@ -292,6 +294,14 @@ int orcwa_gpu_set_workspace_limit(orcwa_gpu_context* ctx, size_t bytes) {
   return ORCWA_OK;
 }
 
+int orcwa_gpu_set_gemm_precision(orcwa_gpu_context* ctx, int precision) {
+  if (!ctx) return fail("orcwa_gpu_set_gemm_precision: null context");
+  if (precision < ORCWA_GEMM_FP64 || precision > ORCWA_GEMM_BF16X6)
+    return fail("orcwa_gpu_set_gemm_precision: unknown precision");
+  ctx->gemm_prec = precision;
+  return ORCWA_OK;
+}
+
 int orcwa_gpu_destroy(orcwa_gpu_context* ctx) {
   if (!ctx) return ORCWA_OK;
   if (ctx->live_plans > 0)
@@ -306,6 +316,8 @@ int orcwa_gpu_destroy(orcwa_gpu_context* ctx) {
 
 int orcwa_plan_destroy(orcwa_plan* p) {
   if (!p) return ORCWA_OK;
+  if (p->live_graphs > 0)
+    return fail("orcwa_plan_destroy: a graph still uses the plan (destroy the graph first)");
   hipSetDevice(p->ctx->device);
   // the buffers go back to the context once the plan's work is done
   hipStreamSynchronize(p->ctx->stream);
@@ -357,6 +369,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   hipSetDevice(ctx->device);
   orcwa_plan* p = new orcwa_plan;
   p->ctx = ctx;
+  p->gemm_prec = ctx->gemm_prec;
   ++ctx->live_plans;
   p->keep = keep;
   p->modes_only = modes_only;
@@ -553,7 +566,12 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   }
   const int islots = L + 3;      // Rt x2, E, H, one LU per cascade step
   const size_t per_inst = (size_t)(7 * lp_max + lu_max + islots) * slot;
-  long long c_fit = (long long)(budget_gb * 1e9 / (double)per_inst);
+  // memory that does not scale with the chunk: per distinct grid the ten h x h
+  // convolution products, per instance the outputs
+  const double fixed =
+      (double)std::max<size_t>(1, grid_index.size()) * (10.0 * h * h + 4.0 * h) * sizeof(cd) +
+      (double)count * ((2.0 * kMaxPol + 16.0) * h * sizeof(double) + 64.0);
+  long long c_fit = (long long)((budget_gb * 1e9 - fixed) / (double)per_inst);
   c_fit = std::max(1LL, std::min((long long)count, c_fit));
   // equal chunks
   const long long nck = (count + c_fit - 1) / c_fit;
@@ -1063,19 +1081,32 @@ static int eig_recover(orcwa_plan* p, const ChunkTables& ct, hipStream_t s) {
   return ORCWA_OK;
 }
 
-int orcwa_plan_execute(orcwa_plan* p) {
-  if (!p) return fail("null plan");
-  HIPCHK(hipSetDevice(p->ctx->device));
-  hipStream_t s = p->ctx->stream;
+__global__ void k_zero_ints(int* a, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    a[i] = 0;
+}
+
+static void zero_ints(hipStream_t s, int* a, long long n) {
+  const int blocks = (int)std::min<long long>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_zero_ints, dim3(std::max(1, blocks)), dim3(256), 0, s, a, n);
+}
+
+// The whole batch on stream s.  graph: the form that a caller captures into a
+// HIP graph -- device work only (status words cleared by a kernel, no events,
+// no host synchronisation, so no in-line eigen recovery: orcwa_graph_sync
+// re-runs a plan eagerly when one of its eigen solves failed).
+static int plan_run(orcwa_plan* p, hipStream_t s, bool graph) {
   const int n = p->n, h = p->h, G = p->G;
   const ConvGeom g = geom_of(p);
   const cd one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   const long long nn = (long long)n * n;
+  auto record = [&](int k) { return graph ? hipSuccess : hipEventRecord(p->ev[k], s); };
 
-  HIPCHK(hipMemsetAsync(p->d_info, 0, p->count * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(p->d_pinfo, 0, std::max<long long>(1, p->nprob_total) * sizeof(int), s));
-  HIPCHK(hipMemsetAsync(p->d_ginfo, 0, std::max(1, G) * sizeof(int), s));
-  HIPCHK(hipEventRecord(p->ev[0], s));
+  zero_ints(s, p->d_info, p->count);
+  zero_ints(s, p->d_pinfo, std::max<long long>(1, p->nprob_total));
+  zero_ints(s, p->d_ginfo, std::max(1, G));
+  HIPCHK(record(0));
 
   // ---- 1. convolution matrices and per-grid products
   if (G > 0) {
@@ -1103,13 +1134,13 @@ int orcwa_plan_execute(orcwa_plan* p) {
                   0, zero, p->dev<cd* const*>(p->t_gS4), 0, 1, h, G);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(p->ev[1], s));
+  HIPCHK(record(1));
   double ms_modes = 0, ms_cascade = 0, ms_rt = 0, ms_eig = 0;
 
   for (int c = 0; c < p->nchunks; ++c) {
     const ChunkTables& ct = p->chunks[c];
     const int np = ct.nprob, nu = ct.nuni, ni = ct.ninst;
-    HIPCHK(hipEventRecord(p->ev[2], s));
+    HIPCHK(record(2));
     // ---- 2a-c. layer modes
     if (np > 0) {
       cd* const* A = p->dev<cd* const*>(ct.pA);
@@ -1137,15 +1168,15 @@ int orcwa_plan_execute(orcwa_plan* p) {
       ws.Z = p->dev<cd**>(ct.pZ);
       ws.X = p->dev<cd**>(ct.pX);
       ws.tau = p->d_tau;
-      HIPCHK(hipEventRecord(p->ev[5], s));
+      HIPCHK(record(5));
       zgeev_batched(s, n, A, p->dev<const cd* const*>(ct.pAorig), Wp, p->d_lam, ws,
                     p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base,
                     p->dev<cd* const*>(ct.pX), np, nullptr, p->ctx->eig_tol);
-      {
+      if (!graph) {
         const int rc = eig_recover(p, ct, s);
         if (rc != ORCWA_OK) return rc;
       }
-      HIPCHK(hipEventRecord(p->ev[6], s));
+      HIPCHK(record(6));
       // QW1 = [E22 W_bottom ; E11 W_top]
       cd* const* QW = p->dev<cd* const*>(ct.pQW);
       if (nq > 0) {
@@ -1172,10 +1203,10 @@ int orcwa_plan_execute(orcwa_plan* p) {
       launch_uniform(s, g, p->dev<const UniParam*>(ct.uni_params), p->d_keffu,
                      p->dev<cd* const*>(ct.uV), p->d_binv, p->d_info, nu);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(p->ev[3], s));
+    HIPCHK(record(3));
     if (p->modes_only) {
-      HIPCHK(hipEventRecord(p->ev[4], s));
-      HIPCHK(hipEventRecord(p->ev[7], s));
+      HIPCHK(record(4));
+      HIPCHK(record(7));
       continue;
     }
 
@@ -1243,14 +1274,14 @@ int orcwa_plan_execute(orcwa_plan* p) {
       // stores its product transposed
       launch_transpose(s, n, C(E), M, ni);
       if (si > 0)
-        zgemm_batched(s, n, n, n, one, C(H), 0, 1, n, 0, C(Rcur), 0, n, 1, 0, one, M, 0, n, 1,
-                      ni);
+        zgemm_batched_prec(p->gemm_prec, s, n, n, n, one, C(H), 0, 1, n, 0, C(Rcur), 0, n, 1, 0,
+                           one, M, 0, n, 1, ni);
       if (!last) {
         // (F - G)^T = B^T + (A R~)^T
         launch_transpose(s, n, C(H), Rnext, ni);
         if (si > 0)
-          zgemm_batched(s, n, n, n, one, C(E), 0, 1, n, 0, C(Rcur), 0, n, 1, 0, one, Rnext, 0, n,
-                        1, ni);
+          zgemm_batched_prec(p->gemm_prec, s, n, n, n, one, C(E), 0, 1, n, 0, C(Rcur), 0, n, 1, 0,
+                             one, Rnext, 0, n, 1, ni);
       }
       zgetrf_batched(s, n, M, piv, p->d_info + ct.i0, nullptr, ni);
       // R_new^T = (F + G)^-T (F - G)^T
@@ -1279,14 +1310,14 @@ int orcwa_plan_execute(orcwa_plan* p) {
       }
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(p->ev[4], s));
+    HIPCHK(record(4));
     // ---- 2f. plane-wave response
     launch_rt(s, g, p->dev<const RtParam*>(ct.rt_params), p->d_rtv,
               p->d_ref + (size_t)ct.i0 * p->maxpol * h, p->d_trn + (size_t)ct.i0 * p->maxpol * h,
               p->d_amp + (size_t)ct.i0 * 8 * h, p->d_info + ct.i0, p->maxpol, ni);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(p->ev[7], s));
-    if (p->nchunks > 1 || c == p->nchunks - 1) {
+    HIPCHK(record(7));
+    if (!graph && (p->nchunks > 1 || c == p->nchunks - 1)) {
       // per-phase timing of this chunk (waits for the chunk; chunks run in order)
       HIPCHK(hipEventSynchronize(p->ev[7]));
       float t = 0;
@@ -1296,6 +1327,10 @@ int orcwa_plan_execute(orcwa_plan* p) {
       if (hipEventElapsedTime(&t, p->ev[4], p->ev[7]) == hipSuccess) ms_rt += t;
     }
   }
+  if (graph) {
+    for (double& v : p->phase_ms) v = -1.0;
+    return ORCWA_OK;
+  }
   float t = 0;
   p->phase_ms[0] = (hipEventElapsedTime(&t, p->ev[0], p->ev[7]) == hipSuccess) ? t : -1;
   p->phase_ms[1] = (hipEventElapsedTime(&t, p->ev[0], p->ev[1]) == hipSuccess) ? t : -1;
@@ -1304,6 +1339,12 @@ int orcwa_plan_execute(orcwa_plan* p) {
   p->phase_ms[4] = ms_rt;
   p->phase_ms[5] = ms_eig;
   return ORCWA_OK;
+}
+
+int orcwa_plan_execute(orcwa_plan* p) {
+  if (!p) return fail("null plan");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  return plan_run(p, p->ctx->stream, false);
 }
 
 int orcwa_plan_sync(orcwa_plan* p) {
@@ -1565,3 +1606,129 @@ void layer_modes_release(LayerModes& m) {
 }
 
 }  // namespace orcwa
+
+// ---------------------------------------------------------------------------
+// Graphs: the executes of several plans captured once into one HIP graph.
+// ---------------------------------------------------------------------------
+struct orcwa_graph {
+  orcwa_gpu_context* ctx = nullptr;
+  std::vector<orcwa_plan*> plans;
+  std::vector<hipStream_t> streams;     // the forks of the capture
+  std::vector<hipEvent_t> events;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int reruns = 0;
+};
+
+int orcwa_graph_destroy(orcwa_graph* gr) {
+  if (!gr) return ORCWA_OK;
+  hipSetDevice(gr->ctx->device);
+  hipStreamSynchronize(gr->ctx->stream);
+  if (gr->exec) hipGraphExecDestroy(gr->exec);
+  if (gr->graph) hipGraphDestroy(gr->graph);
+  for (hipStream_t st : gr->streams) hipStreamDestroy(st);
+  for (hipEvent_t e : gr->events) hipEventDestroy(e);
+  for (orcwa_plan* p : gr->plans) p->live_graphs--;
+  gr->ctx->live_plans--;
+  delete gr;
+  return ORCWA_OK;
+}
+
+int orcwa_graph_create(orcwa_gpu_context* ctx, orcwa_plan* const* plans, int nplans,
+                       int n_streams, orcwa_graph** out) {
+  if (!ctx || !plans || nplans < 1 || !out || n_streams < 1)
+    return fail("orcwa_graph_create: bad argument");
+  *out = nullptr;
+  for (int i = 0; i < nplans; ++i) {
+    if (!plans[i] || plans[i]->ctx != ctx)
+      return fail("orcwa_graph_create: every plan must belong to the context");
+    if (plans[i]->keep || plans[i]->modes_only)
+      return fail("orcwa_graph_create: field and mode plans cannot be captured");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  orcwa_graph* gr = new orcwa_graph;
+  gr->ctx = ctx;
+  gr->plans.assign(plans, plans + nplans);
+  ctx->live_plans++;
+  for (orcwa_plan* p : gr->plans) p->live_graphs++;
+  const int ns = std::min(n_streams, nplans);
+  auto bail = [&](const char* what) {
+    fprintf(stderr, "[openrcwa] orcwa_graph_create: %s\n", what);
+    (void)hipGetLastError();
+    orcwa_graph_destroy(gr);
+    return ORCWA_ERR_HIP;
+  };
+  // forks at the default priority (streams of other priorities must not be mixed)
+  for (int k = 1; k < ns; ++k) {
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+      return bail("stream creation failed");
+    gr->streams.push_back(st);
+  }
+  for (int k = 0; k < 2 * ns; ++k) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bail("event creation failed");
+    gr->events.push_back(e);
+  }
+  hipStream_t s0 = ctx->stream;
+  if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return bail("capture could not begin");
+  int rc = ORCWA_OK;
+  // fork: every extra stream waits on the origin's start
+  hipError_t e = hipEventRecord(gr->events[0], s0);
+  for (int k = 1; k < ns && e == hipSuccess; ++k)
+    e = hipStreamWaitEvent(gr->streams[k - 1], gr->events[0], 0);
+  // plan i on stream i mod ns (the largest plans first spread the work)
+  for (int i = 0; i < nplans && e == hipSuccess && rc == ORCWA_OK; ++i) {
+    hipStream_t st = (i % ns) == 0 ? s0 : gr->streams[i % ns - 1];
+    rc = plan_run(gr->plans[i], st, true);
+  }
+  // join
+  for (int k = 1; k < ns && e == hipSuccess; ++k) {
+    e = hipEventRecord(gr->events[ns + k], gr->streams[k - 1]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s0, gr->events[ns + k], 0);
+  }
+  const hipError_t ec = hipStreamEndCapture(s0, &gr->graph);
+  if (rc != ORCWA_OK || e != hipSuccess || ec != hipSuccess) return bail("capture failed");
+  if (hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0) != hipSuccess)
+    return bail("instantiation failed");
+  *out = gr;
+  return ORCWA_OK;
+}
+
+int orcwa_graph_launch(orcwa_graph* gr) {
+  if (!gr) return fail("orcwa_graph_launch: null graph");
+  HIPCHK(hipSetDevice(gr->ctx->device));
+  HIPCHK(hipGraphLaunch(gr->exec, gr->ctx->stream));
+  return ORCWA_OK;
+}
+
+int orcwa_graph_sync(orcwa_graph* gr) {
+  if (!gr) return fail("orcwa_graph_sync: null graph");
+  HIPCHK(hipSetDevice(gr->ctx->device));
+  hipStream_t s = gr->ctx->stream;
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipGetLastError());
+  // a plan with an eigen solve that failed its residual test (or its QR limit)
+  // is run again eagerly, which takes the recovery branch
+  gr->reruns = 0;
+  for (orcwa_plan* p : gr->plans) {
+    std::vector<int> pin((size_t)std::max<long long>(1, p->nprob_total));
+    HIPCHK(hipMemcpyAsync(pin.data(), p->d_pinfo, pin.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    bool bad = false;
+    for (int v : pin) bad = bad || (v & (ORCWA_STATUS_EIG_RESIDUAL | ORCWA_STATUS_NOCONV));
+    if (bad) {
+      const int rc = plan_run(p, s, false);
+      if (rc != ORCWA_OK) return rc;
+      HIPCHK(hipStreamSynchronize(s));
+      gr->reruns++;
+    }
+  }
+  return ORCWA_OK;
+}
+
+int orcwa_graph_reruns(const orcwa_graph* gr) { return gr ? gr->reruns : -1; }

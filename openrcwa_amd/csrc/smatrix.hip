@@ -149,12 +149,15 @@ void slab_core(hipStream_t s, int n, int batch, const Slots& S, const cd* const*
 // 4-7 bottom (Rud, Tuu, Rdu, Tdd); the result lands in slots 13-16.  Slots
 // 8 M1 = I - rdu Rud / LU, 9-10 [X | Y] = M1^-1 [tdd | rdu Tuu],
 // 11 C2 Tuu = Tuu + Rud Y, 12 scratch.
-void star_core(hipStream_t s, int n, int batch, const Slots& S, int* ipiv, int* perm, int* info) {
+// The nine products run at `prec` (GemmPrec: FP64, or FP32 / split BF16 on
+// MFMA with FP64 accumulation of each K step); the LU and solve stay FP64.
+void star_core(hipStream_t s, int n, int batch, const Slots& S, int* ipiv, int* perm, int* info,
+               int prec) {
   const long long nn = (long long)n * n;
   const cd one = mk(1.0, 0.0), mone = mk(-1.0, 0.0), zero = mk(0.0, 0.0);
   auto G = [&](int dst, int a, int bb, cd alpha, cd beta) {
-    zgemm_batched(s, n, n, n, alpha, S.ctab(a), 0, 1, n, 0, S.ctab(bb), 0, 1, n, 0, beta,
-                  S.tab(dst), 0, 1, n, batch);
+    zgemm_batched_prec(prec, s, n, n, n, alpha, S.ctab(a), 0, 1, n, 0, S.ctab(bb), 0, 1, n, 0,
+                       beta, S.tab(dst), 0, 1, n, batch);
   };
   // C1 = (I - rdu Rud)^-1 (core/RCWAIntegrator.cpp:57-60)
   set_identity_batched(s, n, S.tab(8), batch);
@@ -192,6 +195,7 @@ struct ZWork {
   int* perm = nullptr;
   int* info = nullptr;
   int cap = 0;            // batch capacity of ipiv / perm / info
+  int prec = 0;           // GemmPrec of the star products
   ~ZWork() {
     for (Slots* q : slots) delete q;
   }
@@ -250,7 +254,7 @@ struct ZWork {
           dst.push_back(T->slot(k, 4 + q));
         }
       if ((rc = copies(src, dst))) return rc;
-      star_core(s, n, pairs, *T, ipiv, perm, info);
+      star_core(s, n, pairs, *T, ipiv, perm, info, prec);
       std::vector<Item> next;
       for (int k = 0; k < pairs; ++k)
         next.push_back({{T->slot(k, 13), T->slot(k, 14), T->slot(k, 15), T->slot(k, 16)}});
@@ -451,7 +455,7 @@ int orcwa_gpu_redheffer_star(orcwa_gpu_context* ctx, int n, int batch, const dou
     SCHK(hipMemcpyAsync(S.slot(b, 4), bottom + 2 * 4 * nn * b, sizeof(cd) * 4 * nn,
                         hipMemcpyHostToDevice, s));
   }
-  star_core(s, n, batch, S, d_ipiv, d_perm, d_info);
+  star_core(s, n, batch, S, d_ipiv, d_perm, d_info, ctx->gemm_prec);
   SCHK(hipGetLastError());
   for (int b = 0; b < batch; ++b)
     SCHK(hipMemcpyAsync(out + 2 * 4 * nn * b, S.slot(b, 13), sizeof(cd) * 4 * nn,
@@ -489,6 +493,7 @@ int orcwa_gpu_rcwa_integrate(orcwa_gpu_context* ctx, const orcwa_zstack* zs,
   const double k0 = 2.0 * kPi / zs->lambda, dz = (zs->z1 - zs->z0) / N;
   ZWork w;
   if ((rc = w.init(s, n, N))) return rc;
+  w.prec = ctx->gemm_prec;
   // 1. slab S-matrices of samples 1..N in the basis (W, V) of sample 0
   Slots* S = w.make(11, N, rc);
   if (rc) return rc;
@@ -626,6 +631,7 @@ int orcwa_gpu_redheffer_integrate(orcwa_gpu_context* ctx, const orcwa_zsampler* 
   const int n = sets[0].n;
   ZWork w;
   if ((rc = w.init(s, n, 1))) return rc;
+  w.prec = ctx->gemm_prec;
   struct Seg {
     double a, b;
     int ref, sa, sm, sb;
@@ -647,6 +653,7 @@ int orcwa_gpu_redheffer_integrate(orcwa_gpu_context* ctx, const orcwa_zsampler* 
     const int K = (int)open.size();
     ZWork lw;                 // level-local tables and pivots
     if ((rc = lw.init(s, n, K))) return rc;
+    lw.prec = ctx->gemm_prec;
     // slots: 0 W/LU, 1 V/LU, 2-4 X_i, 5-7 Y_i, 8 P_i - Pr, 9 Q_i - Qr, 10-13 terms
     Slots* T = w.make(14, K, rc);
     if (rc) return rc;
@@ -747,6 +754,7 @@ int orcwa_gpu_redheffer_integrate(orcwa_gpu_context* ctx, const orcwa_zsampler* 
   if (n_segments) *n_segments = M;
   ZWork pw;
   if ((rc = pw.init(s, n, M))) return rc;
+  pw.prec = ctx->gemm_prec;
   // projections onto the previous segment's basis (:242-262).  Slots: 0 W/LU,
   // 1 V/LU, 2 W^-1 leftW, 3 V^-1 leftV, 4 A, 5 B, 6 A - rdu B / LU,
   // 7-8 [rdu A - B | tdd] -> [rdu' | tdd'], 9 B tdd', 10 rud', 11 A + B rdu',

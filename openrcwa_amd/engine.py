@@ -45,11 +45,19 @@ class Context:
         _lib.check(self._lib.orcwa_gpu_set_workspace_limit(self.handle, int(nbytes)),
                    "orcwa_gpu_set_workspace_limit")
 
+    def set_gemm_precision(self, precision):
+        """Precision of the star-product GEMMs of later plans and star products:
+        "fp64", "fp32", "bf16x3" or "bf16x6" (orcwa_gpu_set_gemm_precision)."""
+        prec = _lib.GEMM_PRECISIONS.get(precision, precision)
+        _lib.check(self._lib.orcwa_gpu_set_gemm_precision(self.handle, int(prec)),
+                   "orcwa_gpu_set_gemm_precision")
+
     def close(self):
         """Destroy the context; its outstanding plans are closed first (the C
         ABI refuses to destroy a context with live plans)."""
         if getattr(self, "_h", None):
-            for p in list(self._plans):
+            # graphs first: a plan used by a live graph refuses destruction
+            for p in sorted(list(self._plans), key=lambda x: not isinstance(x, Graph)):
                 p.close()
             _lib.check(self._lib.orcwa_gpu_destroy(self._h), "orcwa_gpu_destroy")
             self._h = None
@@ -369,6 +377,105 @@ def solve_uneven(ctx: Context, stacks, streams: int = 1, pad_layers: bool = True
     if errors:
         raise errors[0]
     return out
+
+
+class Graph:
+    """orcwa_graph_*: the executes of several plans captured once into one HIP
+    graph and replayed with one launch."""
+
+    def __init__(self, ctx: Context, plans, streams: int = 4):
+        self._lib = _lib.load()
+        self.ctx = ctx
+        self.plans = list(plans)
+        arr = (c_void_p * len(self.plans))(*[p_._h.value for p_ in self.plans])
+        h = c_void_p()
+        _lib.check(self._lib.orcwa_graph_create(ctx.handle, arr, len(self.plans), int(streams),
+                                                ctypes.byref(h)), "orcwa_graph_create")
+        self._h = h
+        ctx._plans.add(self)
+
+    def launch(self):
+        _lib.check(self._lib.orcwa_graph_launch(self._h), "orcwa_graph_launch")
+
+    def sync(self):
+        _lib.check(self._lib.orcwa_graph_sync(self._h), "orcwa_graph_sync")
+
+    @property
+    def reruns(self) -> int:
+        return int(self._lib.orcwa_graph_reruns(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.orcwa_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _group_bytes(g) -> float:
+    """Device bytes an unchunked plan of a group would take (per instance: the
+    patterned layers' seven n x n slots, the cascade's L + 3; per distinct grid
+    ten h x h products)."""
+    h = (2 * g["nX"] + 1) * (2 * g["nY"] + 1)
+    n = 2 * h
+    eps = g["eps"]
+    L = eps.shape[1]
+    flat = eps.reshape(eps.shape[0], L, -1)
+    patterned = ~np.all(flat == flat[:, :, :1], axis=2)
+    lp = int(patterned.sum(axis=1).max())
+    slot = 16.0 * n * n
+    return eps.shape[0] * (7 * lp + (L - lp) + L + 3) * slot + \
+        int(patterned.sum()) * 10.0 * 16.0 * h * h
+
+
+class UnevenBatch:
+    """An uneven batch (stacks of different order counts, layer counts, cell
+    layouts and thicknesses, grouped as _uneven_groups does) with every group's
+    plan resident at once and their executes captured in one HIP graph: a
+    solve is then one graph launch, the groups overlapping on `streams` forks.
+    Each plan's workspace is a share of `frac` of the free device memory in
+    proportion to what the group would take unchunked."""
+
+    def __init__(self, ctx: Context, stacks, streams: int = 4, pad_layers: bool = True,
+                 frac: float = 0.8):
+        self.ctx = ctx
+        self.count = len(stacks)
+        groups = list(_uneven_groups(stacks, pad_layers))
+        need = [_group_bytes(g) for g, _ in groups]
+        free = _device_free_bytes(ctx.device)
+        budget = frac * free if free else sum(need)
+        total = sum(need)
+        self.plans, self.index = [], []
+        try:
+            for (g, idx), b in zip(groups, need):
+                ctx.set_workspace_limit(int(b if total <= budget else budget * b / total) + (1 << 20))
+                self.plans.append(_group_plan(ctx, g))
+                self.index.append(idx)
+        finally:
+            ctx.set_workspace_limit(0)
+        self.graph = Graph(ctx, self.plans, streams)
+
+    def run(self):
+        """One solve of the whole batch: a graph launch, then the results as
+        solve_uneven returns them."""
+        self.graph.launch()
+        self.graph.sync()
+        out = [None] * self.count
+        for p_, idx in zip(self.plans, self.index):
+            r = p_.results()
+            for k, i in enumerate(idx):
+                out[i] = (r.ref[k], r.trn[k], int(r.status[k]))
+        return out
+
+    def close(self):
+        self.graph.close()
+        for p_ in self.plans:
+            p_.close()
+        self.plans = []
 
 
 def field_image(ctx: Context, nX, nY, coord_x, coord_y, thickness, lam, kx0, ky0, eps, pol,

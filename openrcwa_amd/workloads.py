@@ -55,6 +55,18 @@ class SplitMix64:
     def uniform(self, lo=0.0, hi=1.0) -> float:
         return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
 
+    def uniforms(self, count: int) -> np.ndarray:
+        """The next `count` uniform() values in [0, 1) at once (the same
+        sequence as count calls)."""
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + k * np.uint64(0x9E3779B97F4A7C15)
+            self.state = int(z[-1]) if count else self.state
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
 
 PILLAR_SEED = 20261202
 

@@ -17,7 +17,8 @@ def declared_functions():
             continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names += re.findall(r"^\s*(?:int|double|void)\s+\*?\s*([a-z_][a-z0-9_]*)\s*\(", text, re.M)
+        names += re.findall(r"^\s*(?:int|double|void|long long)\s+\*?\s*([a-z_][a-z0-9_]*)\s*\(",
+                            text, re.M)
     return sorted(set(names))
 
 
@@ -32,7 +33,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_binding_lists_the_header():
     from openrcwa_amd import _lib
-    assert set(_lib.EXPORTED) <= set(declared_functions())
+    assert set(_lib.EXPORTED) | set(_lib.EXPORTED_SURROGATE) <= set(declared_functions())
+    assert "orcwa_mlp_param_count" in declared_functions()
 
 
 def test_errors_without_device_are_reported():

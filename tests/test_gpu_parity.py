@@ -128,7 +128,7 @@ def test_pillar_workload_reduced_orders(gpu_ctx, nX):
 
 def test_eigen_recovery_on_gpu(gpu_ctx, monkeypatch):
     """The reference's recovery of a failed eigen solve (rcwa/MKLEigenSolver.hpp:
-    81-93, 160-176), forced on every layer by a negative acceptance tolerance:
+    81-93, 160-176), forced on every patterned layer by a negative acceptance tolerance:
     each patterned layer is solved again on A + diag(1e-11 (i+1) max|A|) and
     reported with EIG_PERTURBED (and EIG_RESIDUAL, since the forced test fails
     again).  The per-order results match the oracle taking the same branch
@@ -144,13 +144,28 @@ def test_eigen_recovery_on_gpu(gpu_ctx, monkeypatch):
     finally:
         gh.set_eig_tolerance(gpu_ctx, 1e-8)
     assert (res.status == (_lib.STATUS_EIG_PERTURBED | _lib.STATUS_EIG_RESIDUAL)).all()
-    monkeypatch.setattr(o, "EIG_TOL", -1.0)
+    # the GPU solves homogeneous layers in closed form, so only the patterned
+    # layers take the branch; the oracle is forced the same way (perturbing
+    # the homogeneous layers too moves R/T by ~1e-8 on this batch)
+    solve = o.Layer.solve
+
+    def solve_forced(self, *a, **k):
+        uniform = (np.all(self.eps_grid == self.eps_grid.flat[0]) and
+                   np.all(self.mu_grid == self.mu_grid.flat[0]))
+        saved = o.EIG_TOL
+        o.EIG_TOL = saved if uniform else -1.0
+        try:
+            return solve(self, *a, **k)
+        finally:
+            o.EIG_TOL = saved
+    monkeypatch.setattr(o.Layer, "solve", solve_forced)
     for j, i in enumerate(idx):
         REF, TRN, _ = oracle_pillar(w, i)
         for p in range(2):
             assert np.abs(res.ref[j, p] - REF[p]).max() < TOL
             assert np.abs(res.trn[j, p] - TRN[p]).max() < TOL
-    assert np.abs(res.ref - base.ref).max() < 1e-8 and np.abs(res.trn - base.trn).max() < 1e-8
+    # the perturbation (1e-11 (i + 1) max|A|, up to ~1e-9 max|A| at n = 98) moves R/T by ~2e-8
+    assert np.abs(res.ref - base.ref).max() < 1e-7 and np.abs(res.trn - base.trn).max() < 1e-7
     assert np.abs(res.ref - base.ref).max() > 0.0
 
 
@@ -317,3 +332,24 @@ def test_uneven_growing_mode_is_flagged(gpu_ctx):
         for q in range(2):
             assert np.abs(ref[q] - REF[q]).max() < 1e-9
             assert np.abs(trn[q] - TRN[q]).max() < 1e-9
+
+
+def test_uneven_graph_replays_match_eager(gpu_ctx):
+    """The uneven batch with every group resident and captured in one HIP graph:
+    each replay (with eager solves of the same groups in between) equals the
+    group-by-group eager solve, per order, for every stack."""
+    stacks = workloads.uneven_stacks(48, n_range=(3, 6), layer_range=(1, 5))
+    eager = engine.solve_uneven(gpu_ctx, stacks)
+    ub = engine.UnevenBatch(gpu_ctx, stacks, streams=3)
+    try:
+        assert len(ub.plans) > 1
+        for rep in range(3):
+            got = ub.run()
+            assert ub.graph.reruns == 0
+            for (r1, t1, s1), (r2, t2, s2) in zip(got, eager):
+                assert s1 == s2
+                assert np.abs(r1 - r2).max() < 1e-13 and np.abs(t1 - t2).max() < 1e-13
+            # eager work on the same context between replays
+            engine.solve_uneven(gpu_ctx, stacks[:5])
+    finally:
+        ub.close()

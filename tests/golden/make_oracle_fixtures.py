@@ -4,7 +4,7 @@ at n = 1250 and 1922 takes minutes per stack).  The inputs are regenerated in
 the tests from the seeded workload generators, so the fixture holds only the
 instance descriptors and the oracle's outputs.
 
-  python tests/golden/make_oracle_fixtures.py      (CPU only, ~30 min)
+  python tests/golden/make_oracle_fixtures.py      (CPU only, ~4 min on 8 cores)
 
 Contents (each REF/TRN array is (n_pol = 2, h), TE then TM):
   pillar_idx, pillar_ref, pillar_trn    16 instances of the 2D 15 x 15-order

@@ -353,3 +353,63 @@ def test_uneven_graph_replays_match_eager(gpu_ctx):
             engine.solve_uneven(gpu_ctx, stacks[:5])
     finally:
         ub.close()
+
+
+FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "oracle_large.npz")
+
+
+def _fixture():
+    if not os.path.exists(FIXTURE):
+        pytest.fail("tests/golden/oracle_large.npz is missing: "
+                    "python tests/golden/make_oracle_fixtures.py")
+    return np.load(FIXTURE, allow_pickle=False)
+
+
+def test_pillar_full_size_against_oracle_fixture(gpu_ctx):
+    """16 instances spread over the 2048-solve 15 x 15-order pillar batch
+    (n = 450), solved inside the full batch, per order against the FP64 oracle
+    (tests/golden/oracle_large.npz)."""
+    f = _fixture()
+    w = workloads.pillar2d()
+    res = gpu_pillar(gpu_ctx, w, np.arange(w["lam"].size))
+    assert not res.status.any()
+    for j, i in enumerate(f["pillar_idx"]):
+        for p in range(2):
+            assert np.abs(res.ref[i, p] - f["pillar_ref"][j, p]).max() < 1e-8
+            assert np.abs(res.trn[i, p] - f["pillar_trn"][j, p]).max() < 1e-8
+
+
+def test_uneven_large_stacks_against_oracle_fixture(gpu_ctx):
+    """Uneven-generator stacks at N = 11 and 12 (n = 1058, 1250) with ten
+    layers (Lorentz, lossy metal and patterned layers), in one uneven batch
+    with smaller stacks, per order against the FP64 oracle."""
+    f = _fixture()
+    big = [workloads.uneven_stacks(1, seed=int(sd), n_range=(int(N), int(N)),
+                                   layer_range=(10, 10))[0]
+           for N, sd in zip(f["uneven_n"], f["uneven_seeds"])]
+    stacks = big + workloads.uneven_stacks(6, n_range=(3, 5), layer_range=(1, 4))
+    res = engine.solve_uneven(gpu_ctx, stacks)
+    for k in range(len(big)):
+        ref, trn, st = res[k]
+        assert st & ~engine._lib.STATUS_ILLCOND == 0
+        assert np.abs(ref - f[f"uneven_ref{k}"]).max() < 1e-8
+        assert np.abs(trn - f[f"uneven_trn{k}"]).max() < 1e-8
+
+
+def test_large2d_all_layers_oblique_against_oracle_fixture(gpu_ctx):
+    """The 31 x 31-order grating (n = 1922) with all six patterned layers at
+    theta = 20 deg, phi = 30 deg, per order against the FP64 oracle."""
+    f = _fixture()
+    fr, th, ph = f["large_freq_theta_phi"]
+    w = workloads.large2d()
+    lam = w["lam"][int(fr)]
+    k0 = 2 * math.pi / lam
+    t, p_ = math.radians(th), math.radians(ph)
+    te, tm = ofd.polarisation_basis(th, ph)
+    res = engine.solve_batch(gpu_ctx, w["nX"], w["nY"], w["coord_x"], w["coord_y"],
+                             w["thickness"], [lam], [k0 * math.sin(t) * math.cos(p_)],
+                             [k0 * math.sin(t) * math.sin(p_)], w["eps"], [[te, tm]])
+    assert not res.status.any()
+    for q in range(2):
+        assert np.abs(res.ref[0, q] - f["large_ref"][q]).max() < 1e-8
+        assert np.abs(res.trn[0, q] - f["large_trn"][q]).max() < 1e-8

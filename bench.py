@@ -304,6 +304,11 @@ def main():
                     help="pillar2d: the metric's 15x15-order batch; large2d: 31x31 orders, "
                          "six patterned layers, 64 frequencies (no CPU baseline: one "
                          "oracle solve takes minutes)")
+    ap.add_argument("--tune", action="store_true",
+                    help="pillar2d / large2d: autotune the kernel variants on the batch first "
+                         "(recorded in --tuning-file when given)")
+    ap.add_argument("--tuning-file", default="",
+                    help="file the autotuner appends its choice to")
     ap.add_argument("--graph", action="store_true",
                     help="uneven workload: all groups resident, executed as one captured HIP graph")
     ap.add_argument("--graph-streams", type=int, default=4,
@@ -341,8 +346,19 @@ def main():
     print(f"[bench] workload {args.workload}: {B} solves, h={h}, n={n}, {n_pat} patterned layers",
           file=sys.stderr, flush=True)
     ctx = engine.Context(0)
+    if args.tuning_file:
+        ctx.set_tuning_file(args.tuning_file, append=True)
     plan = engine.Plan(ctx, w["nX"], w["nY"], w["coord_x"], w["coord_y"], w["thickness"],
                        w["lam"], w["kx0"], w["ky0"], w["eps"], w["pols"])
+    tuned = None
+    if args.tune:
+        # time every kernel variant on this batch and keep the fastest
+        # (orcwa_plan_autotune), before the timed region
+        t0 = time.perf_counter()
+        qr, tile = plan.autotune()
+        tuned = {"qr_variant": qr, "gemm_tile": tile, "tuning_s": time.perf_counter() - t0}
+        print(f"[bench] autotune: qr variant {qr}, gemm tile {tile} "
+              f"({tuned['tuning_s']:.1f} s)", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         t0 = time.perf_counter()
         plan.execute()
@@ -388,6 +404,8 @@ def main():
            "phase_ms": phases, "status_nonzero": int((res.status != 0).sum()),
            "alg_tflops_whole_solve": solve_flops(h, n_pat, n_pat + 2) * B / (dt / args.steps) / 1e12,
            "roofline": roof, "polarisation_solves_per_s": 2.0 * value}
+    if tuned is not None:
+        out["autotune"] = tuned
 
     if args.gemm_precision != "fp64":
         plan.close()

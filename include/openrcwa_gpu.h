@@ -201,6 +201,21 @@ int orcwa_graph_sync(orcwa_graph *graph);
 int orcwa_graph_reruns(const orcwa_graph *graph);
 int orcwa_graph_destroy(orcwa_graph *graph);
 
+/* Autotuning (no reference counterpart).  The multishift QR of the eigensolver
+ * has three compiled variants (window / shifts per sweep / workgroups per CU:
+ * 0 = 48 / 12 / 2, 1 = 32 / 8 / 3, 2 = 64 / 16 / 1) and the GEMMs three tiles
+ * for their main block (0 = 64 x 64, 1 = 32 x 64, 2 = 64 x 32).
+ * orcwa_plan_autotune executes the plan with each QR variant, then with each
+ * tile under the fastest QR variant, keeps the fastest combination for the
+ * plan and for later plans of the same layer-matrix size n on the context, and
+ * appends "<arch> <n> <qr> <tile> <ms>" to the context's tuning file when it
+ * is writable; the plan's results are then those of the chosen variant.
+ * orcwa_gpu_set_tuning_file reads such a file (lines of other architectures
+ * are skipped; later lines win) and, with append != 0, records later tuning
+ * results in it.  Plans of sizes not in the table use variant 0 and tile 0. */
+int orcwa_gpu_set_tuning_file(orcwa_gpu_context *ctx, const char *path, int append);
+int orcwa_plan_autotune(orcwa_plan *plan, int *qr_variant, int *gemm_tile);
+
 /* Elapsed GPU time (ms) of named phases of the last execute, measured with HIP
  * events on the plan's stream.  phase: 0 total, 1 convolution matrices,
  * 2 P/Q + eigensolve + V, 3 interfaces + cascade, 4 plane-wave R/T,

@@ -85,7 +85,7 @@ EXPORTED = ["orcwa_gpu_create", "orcwa_gpu_destroy", "orcwa_gpu_set_workspace_li
             "orcwa_gpu_slab_smatrix", "orcwa_gpu_redheffer_star", "orcwa_gpu_field_image",
             "orcwa_gpu_rcwa_integrate", "orcwa_gpu_redheffer_integrate", "rcwa_run",
             "orcwa_graph_create", "orcwa_graph_launch", "orcwa_graph_sync", "orcwa_graph_reruns",
-            "orcwa_graph_destroy"]
+            "orcwa_graph_destroy", "orcwa_gpu_set_tuning_file", "orcwa_plan_autotune"]
 
 # every symbol include/openrcwa_surrogate.h declares
 EXPORTED_SURROGATE = ["orcwa_mlp_create", "orcwa_mlp_destroy", "orcwa_mlp_param_count",
@@ -134,6 +134,8 @@ def load(path: str | None = None):
     lib.orcwa_plan_destroy.argtypes = [c_void_p]
     lib.orcwa_plan_phase_ms.argtypes = [c_void_p, c_int]
     lib.orcwa_plan_phase_ms.restype = c_double
+    lib.orcwa_gpu_set_tuning_file.argtypes = [c_void_p, ctypes.c_char_p, c_int]
+    lib.orcwa_plan_autotune.argtypes = [c_void_p, POINTER(c_int), POINTER(c_int)]
     lib.orcwa_graph_create.argtypes = [c_void_p, POINTER(c_void_p), c_int, c_int,
                                        POINTER(c_void_p)]
     lib.orcwa_graph_launch.argtypes = [c_void_p]

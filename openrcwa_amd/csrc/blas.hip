@@ -559,6 +559,11 @@ static void zgemm_launch(hipStream_t s, int M, int N, int K, cd alpha, const cd*
 #undef ORCWA_GEMM_ARGS
 }
 
+// main-block tile of the GEMMs issued by this thread (set_gemm_tile)
+static thread_local int g_gemm_tile = GEMM_TILE_64x64;
+void set_gemm_tile(int tile) { g_gemm_tile = tile; }
+int gemm_tile() { return g_gemm_tile; }
+
 // rows of a strip narrower than 64: the smallest tile height that covers them
 template <int TN>
 static void zgemm_rows(hipStream_t s, int M, int N, int K, cd alpha, const cd* const* A,
@@ -595,9 +600,19 @@ void zgemm_batched_prec(int prec, hipStream_t s, int M, int N, int K, cd alpha,
   // remaining row and column strips (n = 450: 448 + a 2-wide strip, instead
   // of a 64-wide tile row and column that are 97 % padding)
   const int M0 = (M / 64) * 64, N0 = (N / 64) * 64, mr = M - M0, nr = N - N0;
-  if (M0 > 0 && N0 > 0)
-    zgemm_launch<64, 64>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                         beta, C, offC, rsC, csC, batch, prec);
+  if (M0 > 0 && N0 > 0) {
+    // the main block's tile: the autotuned preference of the calling thread's
+    // current plan (64 x 64 unless set)
+    if (g_gemm_tile == GEMM_TILE_32x64)
+      zgemm_launch<32, 64>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
+                           conjB, beta, C, offC, rsC, csC, batch, prec);
+    else if (g_gemm_tile == GEMM_TILE_64x32)
+      zgemm_launch<64, 32>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
+                           conjB, beta, C, offC, rsC, csC, batch, prec);
+    else
+      zgemm_launch<64, 64>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
+                           conjB, beta, C, offC, rsC, csC, batch, prec);
+  }
   if (mr > 0 && N0 > 0)     // bottom strip
     zgemm_rows<64>(s, mr, N0, K, alpha, A, offA + M0 * rsA, rsA, csA, conjA, B, offB, rsB, csB,
                    conjB, beta, C, offC + M0 * rsC, rsC, csC, batch, prec);

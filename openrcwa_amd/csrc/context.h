@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <map>
+#include <string>
 #include <vector>
 
 struct orcwa_gpu_context {
@@ -30,6 +32,15 @@ struct orcwa_gpu_context {
   // precision of the cascade / star-product GEMMs of later plans and star
   // products (orcwa_gpu_set_gemm_precision; GemmPrec values)
   int gemm_prec = 0;
+  // autotuned kernel variants per layer-matrix size n (orcwa_plan_autotune,
+  // orcwa_gpu_set_tuning_file): QrVariant and GemmTile values
+  struct Tune {
+    int qr = 0, tile = 0;
+    double ms = 0.0;
+  };
+  std::map<int, Tune> tuning;
+  std::string tuning_file;       // appended to by orcwa_plan_autotune when writable
+  bool tuning_append = false;
 };
 
 namespace orcwa {

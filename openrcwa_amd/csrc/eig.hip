@@ -501,643 +501,37 @@ __device__ inline bool small_subdiag(const cd* H, int n, int k, int lo, int hi, 
 }
 
 // ---------------------------------------------------------------------------
-// 3b. windowed small-bulge multishift QR (one workgroup per matrix).
-//
-// A sweep chases QNS single-shift bulges (2x2 unitaries G, as above) as a
-// train through the active block [l, hi].  The chase runs inside a QW x QW
-// window held in LDS; the window's rotations are accumulated into a QW x QW
-// unitary U (LDS) and applied afterwards, with FP64 MFMA, to the rows right of
-// the window, the columns above it and the Schur vectors Z.  A bulge may
-// process pair p only when the bulge ahead of it has already processed
-// p + 2 or later; with that spacing the interleaved train is the same
-// similarity as QNS consecutive single-shift sweeps.  Blocks of at most QW
-// rows are finished entirely in LDS.  Shifts are the eigenvalues of the
-// trailing QNS x QNS submatrix.
+// 3b. windowed small-bulge multishift QR, in three variants of window /
+// shifts / workgroups per CU (qr_ms.inc); the plan's autotuner picks one per
+// matrix size, 48 / 12 / 2 by default.
 // ---------------------------------------------------------------------------
-
-// A 48-wide window with 12 shifts keeps the LDS of one workgroup under 80 KB,
-// so two matrices run on each CU and one's latency-bound chase and window
-// Schur overlap the other's MFMA updates (1.28x over a 64-wide window with 16
-// shifts at one workgroup per CU on n = 450, though each matrix takes more
-// windows and sweeps).
-#ifndef ORCWA_QR_W
+namespace qr_w48 {
 #define ORCWA_QR_W 48
-#endif
-#ifndef ORCWA_QR_NS
 #define ORCWA_QR_NS 12
-#endif
-constexpr int QW = ORCWA_QR_W;   // window (a multiple of 16)
-constexpr int QLD = QW + 1;   // padded LDS leading dimension (bank spread)
-constexpr int QT = QW / 16;   // 16-wide blocks of the window
-constexpr int QNS = ORCWA_QR_NS;
-static_assert(QW % 16 == 0 && QW <= 64 && QNS <= 64, "QR window");
-constexpr int QSCHED = QNS * QW;
-#ifndef ORCWA_AED_NW
-#define ORCWA_AED_NW 24
-#endif
-constexpr int AED_NW = ORCWA_AED_NW;   // early-deflation window
-
-struct QrShared {
-  cd Hw[QLD * QW];      // window / small block (column-major, ld QLD)
-  cd U[QLD * QW];       // accumulated unitary; trailing block for the shifts
-  short sched_b[QSCHED];
-  short sched_p[QSCHED];
-  short pass_start[QSCHED + 1];
-  cd rot_c[QNS], rot_s[QNS];
-  int rot_r[QNS];
-  cd shifts[QNS];
-  int bp[QNS];         // next pair of each bulge (-1: not introduced)
-  int bdone[QNS];
-  cd c, s, sh;
-  cd hv[QW];           // Householder vector (early deflation)
-  cd spike[QW];        // early-deflation spike, reduced to its first entry
-  int nsched, npass, l, ok, ired[4];
-};
-
-__device__ inline void lds_rot_left(cd* B, int r, int j0, int j1, cd c, cd s) {
-  for (int j = j0 + (int)threadIdx.x; j < j1; j += 256) {
-    const cd a = B[r + j * QLD], b = B[r + 1 + j * QLD];
-    B[r + j * QLD] = cmul(c, a) + cmul(s, b);
-    B[r + 1 + j * QLD] = c * b - s * a;
-  }
-}
-__device__ inline void lds_rot_right(cd* B, int col, int i0, int i1, cd c, cd s) {
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
-    const cd p = B[i + col * QLD], q = B[i + (col + 1) * QLD];
-    B[i + col * QLD] = p * c + q * s;
-    B[i + (col + 1) * QLD] = q * conj(c) - p * conj(s);
-  }
-}
-
-__device__ inline bool small_subdiag_lds(const cd* B, int k, int lo, int hi, double smlnum) {
-  const cd h = B[k + (k - 1) * QLD];
-  const double ah = cabs1(h);
-  if (ah <= smlnum) return true;
-  const cd hkk = B[k + k * QLD], hk1 = B[(k - 1) + (k - 1) * QLD];
-  double tst = cabs1(hk1) + cabs1(hkk);
-  if (tst == 0.0) {
-    if (k - 2 >= lo) tst += cabs1(B[(k - 1) + (k - 2) * QLD]);
-    if (k + 1 <= hi) tst += cabs1(B[(k + 1) + k * QLD]);
-  }
-  if (ah <= kUlp * tst) {
-    const double hup = cabs1(B[(k - 1) + k * QLD]);
-    const double ab = fmax(ah, hup), ba = fmin(ah, hup);
-    const double d2 = cabs1(hk1 - hkk);
-    const double aa = fmax(cabs1(hkk), d2), bb = fmin(cabs1(hkk), d2);
-    const double ss = aa + ab;
-    if (ba * (ab / ss) <= fmax(smlnum, kUlp * (bb * (aa / ss)))) return true;
-  }
-  return false;
-}
-
-__device__ inline cd wilkinson_lds(const cd* B, int hi) {
-  cd t = B[hi + hi * QLD];
-  cd u = csqrt_principal(B[(hi - 1) + hi * QLD]) * csqrt_principal(B[hi + (hi - 1) * QLD]);
-  double s = cabs1(u);
-  if (s != 0.0) {
-    cd x = 0.5 * (B[(hi - 1) + (hi - 1) * QLD] - t);
-    double sx = cabs1(x);
-    s = fmax(s, sx);
-    cd xs = (1.0 / s) * x, us = (1.0 / s) * u;
-    cd y = s * csqrt_principal(xs * xs + us * us);
-    if (sx > 0.0) {
-      cd xx = (1.0 / sx) * x;
-      if (xx.re * y.re + xx.im * y.im < 0.0) y = -y;
-    }
-    t = t - u * cdiv(u, x + y);
-  }
-  return t;
-}
-
-// Single-shift QR of the m x m (m <= 64) Hessenberg block B (LDS) to upper
-// triangular form, accumulating the rotations into U when wantu.  Run by
-// wave 0 only: every lane computes the (uniform) deflation point, shift and
-// rotation redundantly, lane j owns column j / row j of the updates, and LDS
-// ordering inside the wave is enforced with wavefront-scope fences.
-
-__device__ __forceinline__ bool wave_small_qr(cd* B, cd* U, int m, bool wantu, int maxit) {
-  const int lane = threadIdx.x & 63;
-  const double smlnum = kSafmin * ((double)m / kUlp);
-  int hi = m - 1, its = 0, tot = 0;
-  while (hi > 0) {
-    const bool flag = (lane >= 1 && lane <= hi) ? small_subdiag_lds(B, lane, 0, hi, smlnum) : false;
-    const unsigned long long mask = __ballot(flag);
-    const int l = mask ? 63 - __clzll(mask) : 0;
-    wave_fence();
-    if (l > 0 && lane == 0) B[l + (l - 1) * QLD] = mk(0.0, 0.0);
-    wave_fence();
-    if (l == hi) { --hi; its = 0; continue; }
-    ++its;
-    if (++tot > maxit) return false;
-    cd sh;
-    if (its % 30 == 10) sh = mk(0.75 * cabs1(B[(l + 1) + l * QLD]), 0.0) + B[l + l * QLD];
-    else if (its % 30 == 20) sh = mk(0.75 * cabs1(B[hi + (hi - 1) * QLD]), 0.0) + B[hi + hi * QLD];
-    else sh = wilkinson_lds(B, hi);
-    for (int k = l; k < hi; ++k) {
-      cd x0, x1;
-      if (k == l) { x0 = B[l + l * QLD] - sh; x1 = B[(l + 1) + l * QLD]; }
-      else { x0 = B[k + (k - 1) * QLD]; x1 = B[(k + 1) + (k - 1) * QLD]; }
-      const double nr = sqrt(norm2(x0) + norm2(x1));
-      cd c = mk(1.0, 0.0), sn = mk(0.0, 0.0);
-      if (nr != 0.0) {
-        const double inv = 1.0 / nr;
-        c = inv * x0;
-        sn = inv * x1;
-      }
-      wave_fence();
-      if (nr != 0.0 && k > l && lane == 0) {
-        B[k + (k - 1) * QLD] = mk(nr, 0.0);
-        B[(k + 1) + (k - 1) * QLD] = mk(0.0, 0.0);
-      }
-      if (lane >= k && lane < m) {
-        const cd a = B[k + lane * QLD], b = B[k + 1 + lane * QLD];
-        B[k + lane * QLD] = cmul(c, a) + cmul(sn, b);
-        B[k + 1 + lane * QLD] = c * b - sn * a;
-      }
-      if (wantu && lane < m) {
-        const cd p = U[lane + k * QLD], q = U[lane + (k + 1) * QLD];
-        U[lane + k * QLD] = p * c + q * sn;
-        U[lane + (k + 1) * QLD] = q * conj(c) - p * conj(sn);
-      }
-      wave_fence();
-      if (lane <= min(k + 2, hi)) {
-        const cd p = B[lane + k * QLD], q = B[lane + (k + 1) * QLD];
-        B[lane + k * QLD] = p * c + q * sn;
-        B[lane + (k + 1) * QLD] = q * conj(c) - p * conj(sn);
-      }
-      wave_fence();
-    }
-  }
-  return true;
-}
-
-// block-wide wrapper: wave 0 works, the result (ok) is shared
-__device__ __forceinline__ bool lds_small_qr(QrShared& S, cd* B, int m, bool wantu, int maxit) {
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const bool ok = wave_small_qr(B, S.U, m, wantu, maxit);
-    if (threadIdx.x == 0) S.ok = ok ? 1 : 0;
-  }
-  __syncthreads();
-  return S.ok != 0;
-}
-
-// X <- U^H X for rows [a, a+wa) of global X (ld), columns [c0, c1).  Each
-// wave owns 16-column blocks; the block's operand is loaded in two halves of
-// 32 rows whose loads are issued together, so two memory latencies are paid
-// per block.  Complex products in three real MFMAs (Gauss): with
-// conj(U) = a + ib and X = x + iy,
-//   P = a (x + y),  Q = (a + b) y,  R = (b - a) x,  re = P - Q,  im = P + R.
-__device__ __forceinline__ void ms_apply_uh_rows(const QrShared& S, cd* X, long long ld, int a, int wa, int c0,
-                                 int c1) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int HALF = QW / 8;
-  for (int j0 = c0 + 16 * wave; j0 < c1; j0 += 64) {
-    const int j = j0 + (lane & 15);
-    v4d accP[QT], accQ[QT], accR[QT];
-#pragma unroll
-    for (int t = 0; t < QT; ++t) {
-      accP[t] = (v4d){0, 0, 0, 0};
-      accQ[t] = (v4d){0, 0, 0, 0};
-      accR[t] = (v4d){0, 0, 0, 0};
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      cd cur[HALF];
-#pragma unroll
-      for (int q = 0; q < HALF; ++q) {
-        const int k = 4 * (HALF * h + q) + (lane >> 4);
-        cur[q] = (k < wa && j < c1) ? X[(a + k) + (long long)j * ld] : mk(0.0, 0.0);
-      }
-#pragma unroll
-      for (int q = 0; q < HALF; ++q) {
-        const int k = 4 * (HALF * h + q) + (lane >> 4);
-        const double x = cur[q].re, y = cur[q].im, xy = x + y;
-#pragma unroll
-        for (int t = 0; t < QT; ++t) {
-          const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // A[i][k] = conj(U[k][i])
-          const double ua = u.re, ub = -u.im;
-          accP[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua, xy, accP[t], 0, 0, 0);
-          accQ[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ua + ub, y, accQ[t], 0, 0, 0);
-          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ub - ua, x, accR[t], 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < QT; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * t + (lane >> 4) + 4 * r;
-        if (i < wa && j < c1)
-          X[(a + i) + (long long)j * ld] = mk(accP[t][r] - accQ[t][r], accP[t][r] + accR[t][r]);
-      }
-  }
-}
-
-// Y <- Y U for rows [r0, r1) of global Y (ld), columns [a, a+wa); same
-// blocking over 16-row blocks and the same three-product form with
-// Y = a + ib, U = x + iy.
-__device__ __forceinline__ void ms_apply_u_cols(const QrShared& S, cd* Y, long long ld, int r0, int r1, int a,
-                                int wa) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int HALF = QW / 8;
-  for (int i0 = r0 + 16 * wave; i0 < r1; i0 += 64) {
-    const int i = i0 + (lane & 15);
-    v4d accP[QT], accQ[QT], accR[QT];
-#pragma unroll
-    for (int t = 0; t < QT; ++t) {
-      accP[t] = (v4d){0, 0, 0, 0};
-      accQ[t] = (v4d){0, 0, 0, 0};
-      accR[t] = (v4d){0, 0, 0, 0};
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      cd cur[HALF];
-#pragma unroll
-      for (int q = 0; q < HALF; ++q) {
-        const int k = 4 * (HALF * h + q) + (lane >> 4);
-        cur[q] = (k < wa && i < r1) ? Y[i + (long long)(a + k) * ld] : mk(0.0, 0.0);
-      }
-#pragma unroll
-      for (int q = 0; q < HALF; ++q) {
-        const int k = 4 * (HALF * h + q) + (lane >> 4);
-        const double ya = cur[q].re, yb = cur[q].im, yab = ya + yb, yba = yb - ya;
-#pragma unroll
-        for (int t = 0; t < QT; ++t) {
-          const cd u = S.U[k + (16 * t + (lane & 15)) * QLD];   // B[k][j] = U[k][j]
-          accP[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ya, u.re + u.im, accP[t], 0, 0, 0);
-          accQ[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(yab, u.im, accQ[t], 0, 0, 0);
-          accR[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(yba, u.re, accR[t], 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < QT; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ii = i0 + (lane >> 4) + 4 * r;
-        const int jj = 16 * t + (lane & 15);
-        if (ii < r1 && jj < wa)
-          Y[ii + (long long)(a + jj) * ld] = mk(accP[t][r] - accQ[t][r], accP[t][r] + accR[t][r]);
-      }
-  }
-}
-
-// load H[a:a+wa, a:a+wa] into S.Hw and reset S.U to the identity (zero padded)
-__device__ __forceinline__ void ms_load_window(QrShared& S, const cd* H, int n, int a, int wa) {
-  // QW*QW entries over 256 threads, loads issued in two batches
-  constexpr int PER = (QW * QW + 511) / 512;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    cd v[PER];
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int e = threadIdx.x + 256 * (PER * h + t);
-      const int i = e % QW, j = e / QW;
-      v[t] = (e < QW * QW && i < wa && j < wa) ? H[(a + i) + (long long)(a + j) * n]
-                                              : mk(0.0, 0.0);
-    }
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int e = threadIdx.x + 256 * (PER * h + t);
-      const int i = e % QW, j = e / QW;
-      if (e < QW * QW) {
-        S.Hw[i + j * QLD] = v[t];
-        S.U[i + j * QLD] = (i == j && i < wa) ? mk(1.0, 0.0) : mk(0.0, 0.0);
-      }
-    }
-  }
-}
-
-__device__ __forceinline__ void ms_store_window(const QrShared& S, cd* H, int n, int a, int wa) {
-  for (int e = threadIdx.x; e < QW * QW; e += 256) {
-    const int i = e % QW, j = e / QW;
-    if (i < wa && j < wa) H[(a + i) + (long long)(a + j) * n] = S.Hw[i + j * QLD];
-  }
-}
-
-// the window's deferred updates: rows right, columns above, Schur vectors
-__device__ __forceinline__ void ms_apply_window(const QrShared& S, cd* H, cd* Z, int n, int a, int wa) {
-  if (a + wa < n) ms_apply_uh_rows(S, H, n, a, wa, a + wa, n);
-  if (a > 0) ms_apply_u_cols(S, H, n, 0, a, a, wa);
-  ms_apply_u_cols(S, Z, n, 0, n, a, wa);
-}
-
-// Householder reflector of x (zlarfg): H^H x = beta e1, H = I - tau v v^H, v0 = 1.
-// Wave 0, lanes own the entries x[i] = B[(r0 + i) + col * QLD], i < len; the
-// reflector is written to hv and the column to (beta, 0, ...).
-__device__ inline cd wave_house_col(cd* B, int r0, int col, int len, cd* hv) {
-  const int lane = threadIdx.x & 63;
-  const cd alpha = B[r0 + col * QLD];
-  double p = (lane >= 1 && lane < len) ? norm2(B[(r0 + lane) + col * QLD]) : 0.0;
-  const double xn2 = wave_sum(p);
-  cd tau = mk(0.0, 0.0), scale = mk(0.0, 0.0), beta = alpha;
-  if (!(xn2 == 0.0 && alpha.im == 0.0)) {
-    const double nrm = sqrt(norm2(alpha) + xn2);
-    const double bt = (alpha.re >= 0.0) ? -nrm : nrm;
-    tau = mk((bt - alpha.re) / bt, -alpha.im / bt);
-    scale = cinv(alpha - mk(bt, 0.0));
-    beta = mk(bt, 0.0);
-  }
-  if (lane < len) hv[lane] = (lane == 0) ? mk(1.0, 0.0) : B[(r0 + lane) + col * QLD] * scale;
-  wave_fence();
-  if (lane < len) B[(r0 + lane) + col * QLD] = (lane == 0) ? beta : mk(0.0, 0.0);
-  wave_fence();
-  return tau;
-}
-
-// T(r0:r0+len, c0:nw) <- (I - tau v v^H)^H T(...) and T(0:nw, r0:r0+len), V(...) <- (...)(I - tau v v^H)
-__device__ inline void wave_house_apply(cd* T, cd* V, int nw, int r0, int len, int c0, cd tau,
-                                        const cd* hv) {
-  const int lane = threadIdx.x & 63;
-  if (lane >= c0 && lane < nw) {
-    cd d = mk(0.0, 0.0);
-    for (int i = 0; i < len; ++i) d += cmul(hv[i], T[(r0 + i) + lane * QLD]);
-    const cd f = conj(tau) * d;
-    for (int i = 0; i < len; ++i) T[(r0 + i) + lane * QLD] -= hv[i] * f;
-  }
-  wave_fence();
-  if (lane < nw) {
-    cd d = mk(0.0, 0.0), e = mk(0.0, 0.0);
-    for (int j = 0; j < len; ++j) {
-      d += T[lane + (r0 + j) * QLD] * hv[j];
-      e += V[lane + (r0 + j) * QLD] * hv[j];
-    }
-    const cd fd = tau * d, fe = tau * e;
-    for (int j = 0; j < len; ++j) {
-      T[lane + (r0 + j) * QLD] -= fd * conj(hv[j]);
-      V[lane + (r0 + j) * QLD] -= fe * conj(hv[j]);
-    }
-  }
-  wave_fence();
-}
-
-// Aggressive early deflation (zlaqr3 mathematics, without its reordering) on
-// the trailing nw x nw window [kw, hi] of the active block [l, hi], kw > l:
-// the window's Schur form T = V^H H_w V in LDS; its spike s conj(V(0, :))
-// (s = H(kw, kw-1)) is tested from the bottom up to the first undeflatable
-// eigenvalue; the undeflated part is returned to Hessenberg form and V is
-// applied to the rest of H and to Z.
-// Returns the number of deflated eigenvalues, or -1 when the window's QR did
-// not converge (H untouched).  S.ok = 1 and S.shifts = the bottom QNS
-// undeflated eigenvalues when there are at least QNS of them.
-__device__ int aed_window(QrShared& S, cd* H, cd* Z, int n, int hi, int nw, double smlnum) {
-  const int kw = hi - nw + 1;
-  const int tid = threadIdx.x, lane = tid & 63;
-  ms_load_window(S, H, n, kw, nw);
-  if (!lds_small_qr(S, S.Hw, nw, true, 30 * nw + 100)) return -1;
-  const cd s = H[kw + (long long)(kw - 1) * n];
-  cd* T = S.Hw;
-  cd* V = S.U;
-  if (tid < 64) {
-    // trailing eigenvalues whose spike entry is negligible deflate; the first
-    // undeflatable one from the bottom ends the test (no reordering)
-    const double as = cabs1(s);
-    int ns = nw;
-    while (ns > 0) {
-      double foo = cabs1(T[(ns - 1) + (ns - 1) * QLD]);
-      if (foo == 0.0) foo = as;
-      if (as * cabs1(V[(ns - 1) * QLD]) <= fmax(smlnum, kUlp * foo)) --ns;
-      else break;
-    }
-    if (ns >= QNS) {
-      if (lane < QNS) S.shifts[lane] = T[(ns - QNS + lane) + (ns - QNS + lane) * QLD];
-    }
-    wave_fence();
-    const cd sp = (ns == 0) ? mk(0.0, 0.0) : s;
-    if (ns < nw || (sp.re == 0.0 && sp.im == 0.0)) {
-      if (ns > 1 && !(sp.re == 0.0 && sp.im == 0.0)) {
-        // reflect the spike s conj(V(0, 0:ns)) onto its first entry, then
-        // return T(0:ns, 0:ns) to Hessenberg form
-        if (lane < ns) S.spike[lane] = sp * conj(V[lane * QLD]);
-        wave_fence();
-        const cd tau = wave_house_col(S.spike, 0, 0, ns, S.hv);
-        wave_house_apply(T, V, nw, 0, ns, 0, tau, S.hv);
-        for (int k = 0; k + 2 < ns; ++k) {
-          const cd t2 = wave_house_col(T, k + 1, k, ns - k - 1, S.hv);
-          wave_house_apply(T, V, nw, k + 1, ns - k - 1, k + 1, t2, S.hv);
-        }
-      }
-      if (lane == 0) S.c = sp * conj(V[0]);   // new spike H(kw, kw-1)
-      if (lane == 0) S.nsched = ns;
-    } else if (lane == 0) {
-      S.nsched = -nw;    // nothing deflated: H unchanged
-    }
-    if (lane == 0) S.ok = (ns >= QNS) ? 1 : 0;
-  }
-  __syncthreads();
-  const int code = S.nsched;
-  if (code < 0) return 0;
-  ms_store_window(S, H, n, kw, nw);
-  if (tid == 0) H[kw + (long long)(kw - 1) * n] = S.c;
-  ms_apply_window(S, H, Z, n, kw, nw);
-  __syncthreads();
-  return nw - code;
-}
-
-#ifndef ORCWA_QR_WG_PER_CU
 #define ORCWA_QR_WG_PER_CU 2
-#endif
-static_assert(sizeof(QrShared) * ORCWA_QR_WG_PER_CU <= 160 * 1024, "QR LDS per CU");
-__global__ __launch_bounds__(256, ORCWA_QR_WG_PER_CU) void k_qr_ms(int n, cd* const* Hp, cd* const* Zp, int* info,
-                                               int maxsweep) {
-  __shared__ QrShared S;
-  const int b = blockIdx.x;
-  cd* __restrict__ H = Hp[b];
-  cd* __restrict__ Z = Zp[b];
-  const int tid = threadIdx.x;
-  const double smlnum = kSafmin * ((double)n / kUlp);
-  int hi = n - 1, sweeps = 0, noprog = 0;
-  while (hi >= 0) {
-    int best = 0;
-    for (int k = 1 + tid; k <= hi; k += 256)
-      if (small_subdiag(H, n, k, 0, hi, smlnum)) best = max(best, k);
-    for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off, 64));
-    if ((tid & 63) == 0) S.ired[tid >> 6] = best;
-    __syncthreads();
-    const int l = max(max(S.ired[0], S.ired[1]), max(S.ired[2], S.ired[3]));
-    __syncthreads();
-    if (l > 0 && tid == 0) H[l + (long long)(l - 1) * n] = mk(0.0, 0.0);
-    __syncthreads();
-    if (l == hi) { --hi; noprog = 0; continue; }
-    const int m = hi - l + 1;
-    if (m <= QW) {
-      ms_load_window(S, H, n, l, m);
-      __syncthreads();
-      const bool ok = lds_small_qr(S, S.Hw, m, true, 30 * m + 100);
-      ms_store_window(S, H, n, l, m);
-      __syncthreads();
-      ms_apply_window(S, H, Z, n, l, m);
-      __syncthreads();
-      if (!ok) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
-      hi = l - 1;
-      noprog = 0;
-      continue;
-    }
-    // aggressive early deflation of the trailing window; a sweep follows
-    // unless it deflated more than 14 % of the window
-    bool aed_shifts = false;
-    {
-      const int nd = aed_window(S, H, Z, n, hi, AED_NW, smlnum);
-      if (nd > 0) {
-        hi -= nd;
-        noprog = 0;
-        if (100 * nd > 14 * AED_NW || hi - l + 1 <= QW) continue;
-      }
-      aed_shifts = (nd >= 0) && (S.ok != 0);
-    }
-    // shifts: the bottom undeflated eigenvalues of the early-deflation
-    // window, else the eigenvalues of the trailing QNS x QNS block (in S.U)
-    constexpr int ns = QNS;
-    if (!aed_shifts) {
-      for (int e = tid; e < QW * QW; e += 256) {
-        const int i = e % QW, j = e / QW;
-        cd v = mk(0.0, 0.0);
-        if (i < ns && j < ns && i <= j + 1)
-          v = H[(hi - ns + 1 + i) + (long long)(hi - ns + 1 + j) * n];
-        S.U[i + j * QLD] = v;
-      }
-      __syncthreads();
-      lds_small_qr(S, S.U, ns, false, 30 * ns + 100);
-    }
-    ++noprog;
-    if (tid < ns) {
-      cd sh = aed_shifts ? S.shifts[tid] : S.U[tid + tid * QLD];
-      if (noprog % 6 == 0) {
-        const double pert = 0.75 * cabs1(H[hi + (long long)(hi - 1) * n]);
-        double sn, cs;
-        sincos(2.0 * 3.14159265358979 * tid / ns, &sn, &cs);
-        sh = sh + mk(pert * cs, pert * sn);
-      }
-      S.shifts[tid] = sh;
-      S.bp[tid] = -1;
-      S.bdone[tid] = 0;
-    }
-    __syncthreads();
-    if (++sweeps > maxsweep) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
-    int a = l;
-    while (true) {
-      const int bend = min(a + QW, hi + 1);
-      const int wa = bend - a;
-      ms_load_window(S, H, n, a, wa);
-      if (tid < 64) {
-        // Schedule of (bulge, pair) moves inside this window, grouped in
-        // passes (the moves of one pass touch disjoint row and column pairs).
-        // Lane i owns bulge i.  Within a pass bulge i moves iff it can move
-        // on its own (A) or it may follow (B) and bulge i-1 moved: a boolean
-        // carry chain solved on ballot masks.  A follower's spacing to its
-        // predecessor is >= 2 pairs, so "predecessor moved" always suffices.
-        const int lane = tid;
-        const int lim = (bend - 1 < hi) ? bend - 3 : hi - 1;
-        int bpv = (lane < QNS) ? S.bp[lane] : 0;
-        int bdv = (lane < QNS) ? S.bdone[lane] : 1;
-        int cnt = 0, np = 0;
-        while (cnt + QNS <= QSCHED) {
-          const int pbp = __shfl_up(bpv, 1, 64), pbd = __shfl_up(bdv, 1, 64);
-          const bool first = (lane == 0);
-          const bool intro = (bpv < 0);
-          const bool own = !bdv && (intro ? (a == l) : (bpv <= lim && bpv >= a + 1));
-          bool A, B;
-          if (intro) {
-            A = first || pbd || pbp >= l + 3;
-            B = !first && pbp >= 0 && pbp + 1 >= l + 3;
-          } else {
-            A = first || pbd || bpv <= pbp - 3;
-            B = true;
-          }
-          const unsigned long long mA = __ballot(own && A), mB = __ballot(own && B);
-          unsigned long long mv = mA;
-#pragma unroll
-          for (int it = 0; it < QNS; ++it) mv = mA | (mB & (mv << 1));
-          if (mv == 0ull) break;
-          if ((mv >> lane) & 1ull) {
-            const int pos = cnt + __popcll(mv & ((1ull << lane) - 1ull));
-            const int pp = intro ? l : bpv;
-            S.sched_b[pos] = (short)(intro ? -(lane + 1) : lane + 1);
-            S.sched_p[pos] = (short)pp;
-            bpv = pp + 1;
-            if (bpv >= hi) bdv = 1;
-          }
-          if (lane == 0) S.pass_start[np] = (short)cnt;
-          cnt += __popcll(mv);
-          ++np;
-        }
-        int cand = (lane < QNS && !bdv) ? (bpv < 0 ? l + 1 : bpv) : (1 << 30);
-        for (int off = 32; off > 0; off >>= 1) cand = min(cand, __shfl_xor(cand, off, 64));
-        if (lane < QNS) { S.bp[lane] = bpv; S.bdone[lane] = bdv; }
-        if (lane == 0) {
-          S.pass_start[np] = (short)cnt;
-          S.nsched = cnt;
-          S.npass = np;
-          S.l = cand;
-        }
-      }
-      __syncthreads();
-      const int cnt = S.nsched, npass = S.npass;
-      const int rlim = min(hi - a + 1, wa);
-      for (int ps = 0; ps < npass; ++ps) {
-        const int e0 = S.pass_start[ps], ne = S.pass_start[ps + 1] - e0;
-        if (tid < ne) {
-          const int e = e0 + tid;
-          const int r = S.sched_p[e] - a;
-          const int code = S.sched_b[e];
-          cd x0, x1;
-          if (code < 0) { x0 = S.Hw[r + r * QLD] - S.shifts[-code - 1]; x1 = S.Hw[(r + 1) + r * QLD]; }
-          else { x0 = S.Hw[r + (r - 1) * QLD]; x1 = S.Hw[(r + 1) + (r - 1) * QLD]; }
-          const double nr = sqrt(norm2(x0) + norm2(x1));
-          cd c = mk(1.0, 0.0), sn = mk(0.0, 0.0);
-          if (nr != 0.0) {
-            const double inv = 1.0 / nr;
-            c = inv * x0;
-            sn = inv * x1;
-            if (code > 0) { S.Hw[r + (r - 1) * QLD] = mk(nr, 0.0); S.Hw[(r + 1) + (r - 1) * QLD] = mk(0.0, 0.0); }
-          }
-          S.rot_c[tid] = c;
-          S.rot_s[tid] = sn;
-          S.rot_r[tid] = r;
-        }
-        __syncthreads();
-        // all left rotations (disjoint row pairs) and the U columns
-        for (int idx = tid; idx < ne * QW; idx += 256) {
-          const int q = idx / QW, j = idx % QW;
-          const int r = S.rot_r[q];
-          const cd c = S.rot_c[q], sn = S.rot_s[q];
-          if (j >= r && j < wa) {
-            const cd x = S.Hw[r + j * QLD], y = S.Hw[r + 1 + j * QLD];
-            S.Hw[r + j * QLD] = cmul(c, x) + cmul(sn, y);
-            S.Hw[r + 1 + j * QLD] = c * y - sn * x;
-          }
-          if (j < wa) {
-            const cd pu = S.U[j + r * QLD], qu = S.U[j + (r + 1) * QLD];
-            S.U[j + r * QLD] = pu * c + qu * sn;
-            S.U[j + (r + 1) * QLD] = qu * conj(c) - pu * conj(sn);
-          }
-        }
-        __syncthreads();
-        // all right rotations (disjoint column pairs)
-        for (int idx = tid; idx < ne * QW; idx += 256) {
-          const int q = idx / QW, i = idx % QW;
-          const int r = S.rot_r[q];
-          if (i < min(r + 3, rlim)) {
-            const cd c = S.rot_c[q], sn = S.rot_s[q];
-            const cd pv = S.Hw[i + r * QLD], qv = S.Hw[i + (r + 1) * QLD];
-            S.Hw[i + r * QLD] = pv * c + qv * sn;
-            S.Hw[i + (r + 1) * QLD] = qv * conj(c) - pv * conj(sn);
-          }
-        }
-        __syncthreads();
-      }
-      ms_store_window(S, H, n, a, wa);
-      ms_apply_window(S, H, Z, n, a, wa);
-      __syncthreads();
-      const int next = S.l;
-      __syncthreads();
-      if (next == (1 << 30)) break;
-      if (cnt == 0) { if (tid == 0) info[b] |= ORCWA_STATUS_NOCONV; return; }
-      a = next - 1;
-    }
-  }
-}
+#include "qr_ms.inc"
+#undef ORCWA_QR_W
+#undef ORCWA_QR_NS
+#undef ORCWA_QR_WG_PER_CU
+}  // namespace qr_w48
+namespace qr_w32 {
+#define ORCWA_QR_W 32
+#define ORCWA_QR_NS 8
+#define ORCWA_QR_WG_PER_CU 3
+#include "qr_ms.inc"
+#undef ORCWA_QR_W
+#undef ORCWA_QR_NS
+#undef ORCWA_QR_WG_PER_CU
+}  // namespace qr_w32
+namespace qr_w64 {
+#define ORCWA_QR_W 64
+#define ORCWA_QR_NS 16
+#define ORCWA_QR_WG_PER_CU 1
+#include "qr_ms.inc"
+#undef ORCWA_QR_W
+#undef ORCWA_QR_NS
+#undef ORCWA_QR_WG_PER_CU
+}  // namespace qr_w64
 
 // ---------------------------------------------------------------------------
 // 4. eigenvectors of the upper triangular T (ztrevc mathematics, right
@@ -1359,7 +753,7 @@ void eig_recover_scatter(hipStream_t s, int n, int m, const int* idx, const cd* 
 
 void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, cd* const* W,
                    cd* lam, const EigWorkspace& ws, int* info, double* resid,
-                   cd* const* scratch, int batch, hipEvent_t* ev, double tol) {
+                   cd* const* scratch, int batch, hipEvent_t* ev, double tol, int qr_variant) {
   if (batch <= 0 || n <= 0) return;
   const cd* const* Ac = (const cd* const*)A;
   if (n >= 3) {
@@ -1403,7 +797,19 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
   if (ev) (void)hipEventRecord(ev[0], s);
   // zero everything below the first subdiagonal (the Householder vectors)
   hipLaunchKernelGGL(k_zero_below_subdiag, dim3(64, batch), dim3(256), 0, s, n, A);
-  hipLaunchKernelGGL(k_qr_ms, dim3(batch), dim3(256), 0, s, n, A, ws.Z, info, 40 + 3 * n);
+  switch (qr_variant) {
+    case QR_W32:
+      hipLaunchKernelGGL(qr_w32::k_qr_ms, dim3(batch), dim3(256), 0, s, n, A, ws.Z, info,
+                         40 + 3 * n);
+      break;
+    case QR_W64:
+      hipLaunchKernelGGL(qr_w64::k_qr_ms, dim3(batch), dim3(256), 0, s, n, A, ws.Z, info,
+                         40 + 3 * n);
+      break;
+    default:
+      hipLaunchKernelGGL(qr_w48::k_qr_ms, dim3(batch), dim3(256), 0, s, n, A, ws.Z, info,
+                         40 + 3 * n);
+  }
   if (ev) (void)hipEventRecord(ev[1], s);
   if (n >= 64)
     trevc_blocked(s, n, Ac, ws.X, batch);

@@ -36,6 +36,12 @@ void zgemm_batched_prec(int prec, hipStream_t s, int M, int N, int K, cd alpha,
                         long long csB, int conjB, cd beta, cd* const* C, long long offC,
                         long long rsC, long long csC, int batch);
 
+// Tile of the multiple-of-64 main block of the GEMMs the calling thread issues
+// (the other tiles cover the edge strips); set per plan by the autotuner.
+enum GemmTile { GEMM_TILE_64x64 = 0, GEMM_TILE_32x64 = 1, GEMM_TILE_64x32 = 2, GEMM_NTILES = 3 };
+void set_gemm_tile(int tile);
+int gemm_tile();
+
 // Batched LU with partial pivoting of n x n column-major matrices (lda = n),
 // in place.  ipiv[b*n + i] = global row swapped with row i.  info[b] gets
 // ORCWA_STATUS_SINGULAR or-ed in when an exactly zero pivot is met
@@ -68,10 +74,13 @@ struct EigWorkspace {
   cd** X;        // [batch] n*n
   cd* tau;       // [batch*n]
 };
+// Variants of the multishift QR (window / shifts / workgroups per CU):
+// QR_W48 48 / 12 / 2 (default), QR_W32 32 / 8 / 3, QR_W64 64 / 16 / 1.
+enum QrVariant { QR_W48 = 0, QR_W32 = 1, QR_W64 = 2, QR_NVARIANTS = 3 };
 void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, cd* const* W,
                    cd* lam, const EigWorkspace& ws, int* info, double* resid,
                    cd* const* scratch, int batch, hipEvent_t* phase_events,
-                   double tol = kEigTol);
+                   double tol = kEigTol, int qr_variant = QR_W48);
 // A[b] = Aorig[b] + diag(1e-11 (i+1) max|Aorig[b]|): the matrix of the
 // reference's recovery solve after a failed residual test
 // (rcwa/MKLEigenSolver.hpp:81-93).

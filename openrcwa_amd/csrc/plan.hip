@@ -146,6 +146,8 @@ struct orcwa_plan {
   // chain at the top of every layer, and the instance's layer data
   bool keep = false;
   int gemm_prec = 0;             // GemmPrec of the cascade products (from the context)
+  int qr_variant = 0;            // QrVariant and GemmTile: from the context's tuning
+  int gemm_tile = 0;             // table for this n (orcwa_plan_autotune)
   // modes-only plans stop after the layer modes (no cascade, no R/T); their
   // layers may each have their own cell edges (one edge set per grid)
   bool modes_only = false;
@@ -380,6 +382,13 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
   p->ny = ny;
   p->h = nx * ny;
   p->n = 2 * p->h;
+  {
+    auto t = ctx->tuning.find(p->n);
+    if (t != ctx->tuning.end()) {
+      p->qr_variant = t->second.qr;
+      p->gemm_tile = t->second.tile;
+    }
+  }
   p->L = st->n_layers;
   p->count = count;
   p->ncx = st->n_cx;
@@ -1070,7 +1079,7 @@ static int eig_recover(orcwa_plan* p, const ChunkTables& ct, hipStream_t s) {
     double* res_k = reinterpret_cast<double*>(buf + o_res);
     eig_perturb_batched(s, n, tAo, tA, m);
     zgeev_batched(s, n, tA, tAo, tW, lam_k, ws, info_k, res_k, ws.X, m, nullptr,
-                  p->ctx->eig_tol);
+                  p->ctx->eig_tol, p->qr_variant);
     eig_recover_scatter(s, n, m, reinterpret_cast<const int*>(buf + o_idx), lam_k, info_k,
                         res_k, p->d_lam, p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base);
     e = hipGetLastError();
@@ -1097,6 +1106,12 @@ static void zero_ints(hipStream_t s, int* a, long long n) {
 // no host synchronisation, so no in-line eigen recovery: orcwa_graph_sync
 // re-runs a plan eagerly when one of its eigen solves failed).
 static int plan_run(orcwa_plan* p, hipStream_t s, bool graph) {
+  // the plan's GEMM tile for the launches of this thread, restored on return
+  struct TileGuard {
+    int saved;
+    explicit TileGuard(int t) : saved(gemm_tile()) { set_gemm_tile(t); }
+    ~TileGuard() { set_gemm_tile(saved); }
+  } tile_guard(p->gemm_tile);
   const int n = p->n, h = p->h, G = p->G;
   const ConvGeom g = geom_of(p);
   const cd one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
@@ -1171,7 +1186,7 @@ static int plan_run(orcwa_plan* p, hipStream_t s, bool graph) {
       HIPCHK(record(5));
       zgeev_batched(s, n, A, p->dev<const cd* const*>(ct.pAorig), Wp, p->d_lam, ws,
                     p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base,
-                    p->dev<cd* const*>(ct.pX), np, nullptr, p->ctx->eig_tol);
+                    p->dev<cd* const*>(ct.pX), np, nullptr, p->ctx->eig_tol, p->qr_variant);
       if (!graph) {
         const int rc = eig_recover(p, ct, s);
         if (rc != ORCWA_OK) return rc;
@@ -1732,3 +1747,96 @@ int orcwa_graph_sync(orcwa_graph* gr) {
 }
 
 int orcwa_graph_reruns(const orcwa_graph* gr) { return gr ? gr->reruns : -1; }
+
+// ---------------------------------------------------------------------------
+// Autotuning of the kernel variants per layer-matrix size.
+// ---------------------------------------------------------------------------
+static std::string arch_name(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown";
+  std::string a = prop.gcnArchName;
+  const size_t c = a.find(':');
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
+int orcwa_gpu_set_tuning_file(orcwa_gpu_context* ctx, const char* path, int append) {
+  if (!ctx || !path) return fail("orcwa_gpu_set_tuning_file: null argument");
+  ctx->tuning_file = path;
+  ctx->tuning_append = append != 0;
+  FILE* f = fopen(path, "r");
+  if (!f) return ORCWA_OK;     // nothing tuned yet
+  const std::string arch = arch_name(ctx->device);
+  char a[64];
+  int n, qr, tile;
+  double ms;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    if (sscanf(line, "%63s %d %d %d %lf", a, &n, &qr, &tile, &ms) != 5) continue;
+    if (arch != a || n < 1 || qr < 0 || qr >= QR_NVARIANTS || tile < 0 || tile >= GEMM_NTILES)
+      continue;
+    orcwa_gpu_context::Tune t;
+    t.qr = qr;
+    t.tile = tile;
+    t.ms = ms;
+    ctx->tuning[n] = t;        // later lines win
+  }
+  fclose(f);
+  return ORCWA_OK;
+}
+
+int orcwa_plan_autotune(orcwa_plan* p, int* qr_variant, int* gemm_tile) {
+  if (!p) return fail("orcwa_plan_autotune: null plan");
+  if (p->keep || p->modes_only) return fail("orcwa_plan_autotune: field and mode plans");
+  if (p->live_graphs > 0) return fail("orcwa_plan_autotune: the plan is captured in a graph");
+  HIPCHK(hipSetDevice(p->ctx->device));
+  auto timed = [&](int qr, int tile, double& ms) -> int {
+    p->qr_variant = qr;
+    p->gemm_tile = tile;
+    const int rc = plan_run(p, p->ctx->stream, false);
+    if (rc != ORCWA_OK) return rc;
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    ms = p->phase_ms[0];
+    return ORCWA_OK;
+  };
+  // the QR variants with the default tile, then the tiles with the best QR
+  int best_qr = QR_W48, best_tile = GEMM_TILE_64x64;
+  double best = 1e300, ms = 0.0;
+  for (int q = 0; q < QR_NVARIANTS; ++q) {
+    const int rc = timed(q, GEMM_TILE_64x64, ms);
+    if (rc != ORCWA_OK) return rc;
+    if (ms > 0.0 && ms < best) {
+      best = ms;
+      best_qr = q;
+    }
+  }
+  for (int t = 1; t < GEMM_NTILES; ++t) {
+    const int rc = timed(best_qr, t, ms);
+    if (rc != ORCWA_OK) return rc;
+    if (ms > 0.0 && ms < best) {
+      best = ms;
+      best_tile = t;
+    }
+  }
+  p->qr_variant = best_qr;
+  p->gemm_tile = best_tile;
+  orcwa_gpu_context::Tune t;
+  t.qr = best_qr;
+  t.tile = best_tile;
+  t.ms = best;
+  p->ctx->tuning[p->n] = t;
+  if (p->ctx->tuning_append && !p->ctx->tuning_file.empty()) {
+    if (FILE* f = fopen(p->ctx->tuning_file.c_str(), "a")) {
+      fprintf(f, "%s %d %d %d %.3f\n", arch_name(p->ctx->device).c_str(), p->n, best_qr,
+              best_tile, best);
+      fclose(f);
+    }
+  }
+  if (qr_variant) *qr_variant = best_qr;
+  if (gemm_tile) *gemm_tile = best_tile;
+  // the last timed execute used another variant: leave the plan's results those of the choice
+  const int rc = plan_run(p, p->ctx->stream, false);
+  if (rc != ORCWA_OK) return rc;
+  HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  return ORCWA_OK;
+}

@@ -9,6 +9,7 @@ sol/rcwa_bridge.cpp:15-16; micrometres on the .orcwa path).
 from __future__ import annotations
 
 import ctypes
+import os
 import weakref
 from ctypes import POINTER, c_double, c_void_p
 from dataclasses import dataclass
@@ -16,6 +17,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
+
+# autotuned kernel variants per layer-matrix size (orcwa_plan_autotune output)
+TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.txt")
 
 
 def _dptr(a: np.ndarray):
@@ -32,6 +36,9 @@ class Context:
         h = c_void_p()
         _lib.check(self._lib.orcwa_gpu_create(ctypes.byref(h), int(device)), "orcwa_gpu_create")
         self._h = h
+        if os.path.exists(TUNING_FILE):
+            # the committed table of autotuned kernel variants (read only)
+            self.set_tuning_file(TUNING_FILE)
 
     @property
     def handle(self):
@@ -44,6 +51,13 @@ class Context:
         (orcwa_gpu_set_workspace_limit; 0 restores the default)."""
         _lib.check(self._lib.orcwa_gpu_set_workspace_limit(self.handle, int(nbytes)),
                    "orcwa_gpu_set_workspace_limit")
+
+    def set_tuning_file(self, path, append=False):
+        """Read autotuned kernel variants from `path` (and append later tuning
+        results to it when append) -- orcwa_gpu_set_tuning_file."""
+        _lib.check(self._lib.orcwa_gpu_set_tuning_file(self.handle, str(path).encode(),
+                                                       1 if append else 0),
+                   "orcwa_gpu_set_tuning_file")
 
     def set_gemm_precision(self, precision):
         """Precision of the star-product GEMMs of later plans and star products:
@@ -178,6 +192,14 @@ class Plan:
 
     def sync(self):
         _lib.check(self._lib.orcwa_plan_sync(self._h), "orcwa_plan_sync")
+
+    def autotune(self):
+        """Time the plan with each kernel variant and keep the fastest
+        (orcwa_plan_autotune).  Returns (qr_variant, gemm_tile)."""
+        q, t = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.orcwa_plan_autotune(self._h, ctypes.byref(q), ctypes.byref(t)),
+                   "orcwa_plan_autotune")
+        return q.value, t.value
 
     def phase_ms(self, phase: int) -> float:
         return float(self._lib.orcwa_plan_phase_ms(self._h, int(phase)))

@@ -40,7 +40,11 @@ def solve_flops(h, n_pat, n_layers):
 
 def _cpu_init():
     # one BLAS thread per worker process: the oracle's numpy/LAPACK calls then
-    # run the reference's single-threaded sequence (sol/rcwa_bridge.cpp:244)
+    # run the reference's single-threaded sequence (sol/rcwa_bridge.cpp:244).
+    # The oracle (numpy and scipy.linalg, each with its own BLAS) is imported
+    # first so that the limit reaches every BLAS library the worker uses; the
+    # pool's environment already asks for one thread.
+    from oracle import rcwa_oracle  # noqa: F401
     from threadpoolctl import threadpool_limits
     global _CPU_LIMIT
     _CPU_LIMIT = threadpool_limits(1)
@@ -64,16 +68,28 @@ def cpu_baseline(w, sample_idx, cores, runs):
     time is the baseline (BASELINE.md protocol: warm-up, median of >= 5)."""
     import multiprocessing as mp
     ctx = mp.get_context("spawn")
-    with ctx.Pool(cores, initializer=_cpu_init) as pool:
-        pool.map(_cpu_worker, [(w, int(sample_idx[k % len(sample_idx)])) for k in range(cores)],
-                 chunksize=1)
-        walls, per = [], []
-        out = None
-        for _ in range(runs):
-            t0 = time.perf_counter()
-            out = pool.map(_cpu_worker, [(w, int(i)) for i in sample_idx], chunksize=1)
-            walls.append(time.perf_counter() - t0)
-            per.extend(o_[3] for o_ in out)
+    threads = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+    saved = {k: os.environ.get(k) for k in threads}
+    for k in threads:
+        os.environ[k] = "1"           # inherited by the spawned workers
+    try:
+        with ctx.Pool(cores, initializer=_cpu_init) as pool:
+            pool.map(_cpu_worker, [(w, int(sample_idx[k % len(sample_idx)]))
+                                   for k in range(cores)], chunksize=1)
+            walls, per = [], []
+            out = None
+            for r in range(runs):
+                t0 = time.perf_counter()
+                out = pool.map(_cpu_worker, [(w, int(i)) for i in sample_idx], chunksize=1)
+                walls.append(time.perf_counter() - t0)
+                per.extend(o_[3] for o_ in out)
+                print(f"[bench] cpu run {r}: {walls[-1]:.2f} s", file=sys.stderr, flush=True)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     return out, walls, per
 
 

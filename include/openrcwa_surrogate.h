@@ -96,6 +96,22 @@ int orcwa_mlp_predict(orcwa_mlp *net, const float *x, long long count, float *me
 int orcwa_mlp_predict_device(orcwa_mlp *net, const float *x, long long count, float *mean,
                              float *err);
 
+/* orcwa_mlp_predict_device without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops): the
+ * padded inputs, activations and member outputs of each pass live in a static
+ * 64 MiB device buffer that every GPU holds until the process ends.  Rows are
+ * processed in passes of as many as fit, 4 (n_in + 2 members width + members
+ * n_out) bytes per row with n_in and n_out padded (count is not limited; every
+ * network orcwa_mlp_create accepts fits at least 448 rows, and a pass of fewer
+ * than 64 rows would be refused with ORCWA_ERR_ARG).  Every call on one GPU
+ * shares that GPU's buffer: calls that
+ * overlap on one GPU race (from different threads or contexts), and the buffer
+ * keeps the last call's contents.  Asynchronous on the context stream; the
+ * network's parameters and error scales must already exist (orcwa_mlp_create).
+ * No counterpart in the reference. */
+int orcwa_mlp_predict_device_noalloc(orcwa_mlp *net, const float *x, long long count, float *mean,
+                                     float *err);
+
 /* GPU time in ms of the last orcwa_mlp_train / orcwa_mlp_predict (HIP events on
  * the context stream). */
 double orcwa_mlp_last_ms(const orcwa_mlp *net);

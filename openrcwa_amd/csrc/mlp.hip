@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -996,6 +997,80 @@ int orcwa_mlp_predict_device(orcwa_mlp* n, const float* x, long long count, floa
     hipLaunchKernelGGL(k_ensemble, dim3((rows + 255) / 256), dim3(256), 0, s, n->d_Yp,
                        n->members, (long long)rows, n->nout, n->nout_pad, n->d_escale,
                        mean + r0 * n->nout, err + r0);
+  }
+  MCHK(hipGetLastError());
+  return ORCWA_OK;
+}
+
+}  // extern "C"
+
+// Inference scratch of orcwa_mlp_predict_device_noalloc: the padded inputs,
+// the two activation buffers and the members' outputs of one pass, in a
+// static device array (every GPU the library runs on holds its own copy until
+// the process ends).  Host code never uses the array's name as an address:
+// its device address comes from hipGetSymbolAddress, once per device.
+static constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// Device address of `scratch` on `device` (current); 0 on success.
+int scratch_address(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return fail("scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      fprintf(stderr, "[openrcwa] hipGetSymbolAddress(scratch) failed: %s\n",
+              hipGetErrorName(e));
+      return ORCWA_ERR_HIP;
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return ORCWA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orcwa_mlp_predict_device_noalloc(orcwa_mlp* n, const float* x, long long count, float* mean,
+                                     float* err) {
+  if (!n || !x || !mean || !err || count < 1)
+    return fail("orcwa_mlp_predict_device_noalloc: bad argument");
+  MCHK(hipSetDevice(n->ctx->device));
+  void* base = nullptr;
+  int rc = scratch_address(n->ctx->device, &base);
+  if (rc) return rc;
+  // rows per pass: inputs [R][nin_pad], activations 2 x [members][R][width],
+  // outputs [members][R][nout_pad], R a multiple of 64 (every part then stays
+  // 256-byte aligned)
+  const size_t per_row = sizeof(float) * ((size_t)n->nin_pad + 2 * (size_t)n->members * n->width +
+                                          (size_t)n->members * n->nout_pad);
+  const long long R = (long long)(kNoAllocScratchBytes / per_row) / 64 * 64;
+  if (R < 64)
+    return fail("orcwa_mlp_predict_device_noalloc: the network needs more than the 64 MiB "
+                "inference scratch for 64 rows (fewer members or a narrower network)");
+  float* inp = static_cast<float*>(base);
+  float* P0 = inp + R * n->nin_pad;
+  float* P1 = P0 + R * n->members * n->width;
+  float* Yp = P1 + R * n->members * n->width;
+  hipStream_t s = n->ctx->stream;
+  for (long long r0 = 0; r0 < count; r0 += R) {
+    const int rows = (int)std::min<long long>(R, count - r0);
+    const long long tot = (long long)rows * n->nin_pad;
+    hipLaunchKernelGGL(k_pad_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       x + r0 * n->nin, (long long)rows, n->nin, n->nin_pad, inp);
+    forward(n, s, inp, 0, rows, false, Yp, n->precision == ORCWA_MLP_BF16, P0, P1);
+    hipLaunchKernelGGL(k_ensemble, dim3((rows + 255) / 256), dim3(256), 0, s, Yp, n->members,
+                       (long long)rows, n->nout, n->nout_pad, n->d_escale, mean + r0 * n->nout,
+                       err + r0);
   }
   MCHK(hipGetLastError());
   return ORCWA_OK;

@@ -72,6 +72,12 @@ struct ChunkTables {
   std::vector<size_t> LU;        // per step: LU of (F + G)^T
   long long prob_base = 0;       // global index of the chunk's first problem
   std::vector<StepTables> steps;
+  // host copies for the interfaces of recovered eigen solves: layer of each
+  // problem, problem of each (instance, layer) (-1 homogeneous), V and keff
+  // of each (instance, layer), H slot of each instance, LU slot of each
+  // (instance, step)
+  std::vector<int> prob_layer, pidx;
+  std::vector<cd*> hV, hK, hH, hLU;
 };
 
 }  // namespace
@@ -789,6 +795,7 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
           if (g >= 0) {
             if ((p->mgid[il] >= 0) != (pass == 1)) continue;
             pidx[(size_t)ii * L + l] = (int)pp.size();
+            ct.prob_layer.push_back(l);
             ProbParam q;
             q.kx0 = p->kx0[i];
             q.ky0 = p->ky0[i];
@@ -890,6 +897,15 @@ static int plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcw
       int u = uidx[(size_t)ii * L + l];
       return u >= 0 ? p->d_binv + (size_t)u * 4 * h : nullptr;
     };
+    ct.pidx.assign(pidx.begin(), pidx.end());
+    for (int ii = 0; ii < ct.ninst; ++ii) {
+      for (int l = 0; l < L; ++l) {
+        ct.hV.push_back(Vof(ii, l));
+        ct.hK.push_back(Kof(ii, l));
+      }
+      ct.hH.push_back(islot(ii, 3));
+      for (int si = 0; si < L - 1; ++si) ct.hLU.push_back(islot(ii, 4 + si));
+    }
     if (keep) {
       p->kE_src = islot(0, 2);
       p->kH_src = islot(0, 3);
@@ -1030,8 +1046,10 @@ int orcwa_plan_create(orcwa_gpu_context* ctx, const orcwa_stack* st, const orcwa
 // test failed (or whose QR did not converge) are solved again on
 // A + diag(1e-11 (i+1) max|A|) and tested against the unperturbed A.  The
 // status bits are read back once per chunk; the recovery itself is rare.
-static int eig_recover(orcwa_plan* p, const ChunkTables& ct, hipStream_t s) {
+static int eig_recover(orcwa_plan* p, const ChunkTables& ct, hipStream_t s,
+                       std::vector<int>& recovered) {
   const int np = ct.nprob, n = p->n;
+  recovered.clear();
   if (np <= 0) return ORCWA_OK;
   std::vector<int> pin(np);
   HIPCHK(hipMemcpyAsync(pin.data(), p->d_pinfo + ct.prob_base, np * sizeof(int),
@@ -1041,6 +1059,7 @@ static int eig_recover(orcwa_plan* p, const ChunkTables& ct, hipStream_t s) {
   for (int q = 0; q < np; ++q)
     if (pin[q] & (ORCWA_STATUS_EIG_RESIDUAL | ORCWA_STATUS_NOCONV)) bad.push_back(q);
   if (bad.empty()) return ORCWA_OK;
+  recovered = bad;
   const int m = (int)bad.size();
   // the problems' slots, read from the host image of the plan's tables
   auto slot = [&](size_t off, int q) {
@@ -1082,6 +1101,71 @@ static int eig_recover(orcwa_plan* p, const ChunkTables& ct, hipStream_t s) {
                   p->ctx->eig_tol, p->qr_variant);
     eig_recover_scatter(s, n, m, reinterpret_cast<const int*>(buf + o_idx), lam_k, info_k,
                         res_k, p->d_lam, p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  HIPCHK(e);
+  return ORCWA_OK;
+}
+
+__global__ void k_scale_rows_keff(int n, cd* const* Hp, const cd* const* Kp) {
+  cd* H = Hp[blockIdx.y];
+  const cd* kf = Kp[blockIdx.y];
+  const long long count = (long long)n * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x)
+    H[e] = H[e] * kf[e % n];
+}
+
+// The interface below a non-magnetic patterned layer a is formed as
+// H = K_a^-1 W_a^-1 P_a V_b, which equals V_a^-1 V_b when W_a, K_a are the
+// eigen solve of P_a Q_a.  A recovered solve is that of A + diag(1e-11 (i+1)
+// max|A|) instead, so for the instances whose layer a was recovered H is
+// formed as the reference forms it, V_a^-1 V_b with an LU of V_a
+// (rcwa/RCWASolver.cpp:537-543), and stored as K_a H to keep the scaling the
+// cascade applies.
+static int interface_fixup(orcwa_plan* p, const ChunkTables& ct, int si,
+                           const std::vector<int>& recovered, hipStream_t s) {
+  const int L = p->L, n = p->n, a = L - 2 - si, b = L - 1 - si;
+  std::vector<int> sel;
+  for (int q : recovered) {
+    if (q >= ct.nprob - ct.nprob_mag) continue;   // magnetic layers use V_a^-1 V_b already
+    if (ct.prob_layer[q] != a) continue;
+    const int i = p->prob_inst[(size_t)ct.prob_base + q];
+    sel.push_back(i - ct.i0);
+  }
+  if (sel.empty()) return ORCWA_OK;
+  const int m = (int)sel.size();
+  std::vector<cd*> tabs((size_t)5 * m);
+  std::vector<int> inst(m);
+  for (int k = 0; k < m; ++k) {
+    const int ii = sel[k];
+    tabs[k] = ct.hV[(size_t)ii * L + a];
+    tabs[m + k] = ct.hV[(size_t)ii * L + b];
+    tabs[2 * m + k] = ct.hLU[(size_t)ii * (L - 1) + si];
+    tabs[3 * m + k] = ct.hH[ii];
+    tabs[4 * m + k] = ct.hK[(size_t)ii * L + a];
+    inst[k] = ct.i0 + ii;
+  }
+  const size_t o_inst = align256(tabs.size() * sizeof(cd*));
+  const size_t bytes = o_inst + m * sizeof(int);
+  char* buf = nullptr;
+  if (hipMalloc(&buf, bytes) != hipSuccess) return ORCWA_ERR_NOMEM;
+  std::vector<char> img(bytes, 0);
+  std::memcpy(img.data(), tabs.data(), tabs.size() * sizeof(cd*));
+  std::memcpy(img.data() + o_inst, inst.data(), m * sizeof(int));
+  hipError_t e = hipMemcpyAsync(buf, img.data(), bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    cd* const* t = reinterpret_cast<cd* const*>(buf);
+    const cd* const* tc = reinterpret_cast<const cd* const*>(buf);
+    const long long nn = (long long)n * n;
+    copy_batched(s, nn, tc, t + 2 * m, m);       // V_a -> LU slot
+    copy_batched(s, nn, tc + m, t + 3 * m, m);   // V_b -> H slot
+    zgetrf_batched(s, n, t + 2 * m, p->d_ipiv, p->d_info, reinterpret_cast<const int*>(buf + o_inst),
+                   m);
+    zgetrs_batched(s, n, n, tc + 2 * m, p->d_ipiv, t + 3 * m, m, p->d_perm);
+    hipLaunchKernelGGL(k_scale_rows_keff, dim3(64, m), dim3(256), 0, s, n, t + 3 * m, tc + 4 * m);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1152,9 +1236,11 @@ static int plan_run(orcwa_plan* p, hipStream_t s, bool graph) {
   HIPCHK(record(1));
   double ms_modes = 0, ms_cascade = 0, ms_rt = 0, ms_eig = 0;
 
+  std::vector<int> recovered;   // problems of the chunk whose eigen solve was recovered
   for (int c = 0; c < p->nchunks; ++c) {
     const ChunkTables& ct = p->chunks[c];
     const int np = ct.nprob, nu = ct.nuni, ni = ct.ninst;
+    recovered.clear();
     HIPCHK(record(2));
     // ---- 2a-c. layer modes
     if (np > 0) {
@@ -1188,7 +1274,7 @@ static int plan_run(orcwa_plan* p, hipStream_t s, bool graph) {
                     p->d_pinfo + ct.prob_base, p->d_resid + ct.prob_base,
                     p->dev<cd* const*>(ct.pX), np, nullptr, p->ctx->eig_tol, p->qr_variant);
       if (!graph) {
-        const int rc = eig_recover(p, ct, s);
+        const int rc = eig_recover(p, ct, s, recovered);
         if (rc != ORCWA_OK) return rc;
       }
       HIPCHK(record(6));
@@ -1272,6 +1358,10 @@ static int plan_run(orcwa_plan* p, hipStream_t s, bool graph) {
         copy_batched(s, nn, p->dev<const cd* const*>(q.Vb), p->dev<cd* const*>(q.H), q.count);
         zgetrf_batched(s, n, qLU, p->d_ipiv, p->d_info, p->dev<const int*>(q.inst), q.count);
         zgetrs_batched(s, n, n, C(qLU), p->d_ipiv, p->dev<cd* const*>(q.H), q.count, p->d_perm);
+      }
+      if (!recovered.empty()) {
+        const int rc = interface_fixup(p, ct, si, recovered, s);
+        if (rc != ORCWA_OK) return rc;
       }
       // A = E + H (in E), B = E - H (in H)
       launch_sum_diff(s, n, E, H, p->dev<const cd* const*>(st.keff_a), ni);

@@ -106,7 +106,14 @@ def bench_uneven(args):
     ctx = engine.Context(0)
     print(f"[bench] workload uneven: {len(stacks)} stacks", file=sys.stderr, flush=True)
     if args.graph:
-        return bench_uneven_graph(args, ctx, stacks)
+        # The full 1024-stack batch as one graph (10 resident plans, about 70k
+        # captured kernels) ended the process with SIGSEGV at its first replay
+        # after capture and instantiation had succeeded; the cause is not
+        # known, so this mode is refused until it is (the 48-stack graph of
+        # tests/test_gpu_parity.py is what is run).
+        print("bench.py: --graph on the full uneven batch is disabled (its first replay "
+              "ended the process with SIGSEGV, cause not yet found)", file=sys.stderr)
+        sys.exit(2)
     for _ in range(args.warmup):
         engine.solve_uneven(ctx, stacks, streams=args.streams, pad_layers=not args.no_pad_layers)
     t0 = time.perf_counter()

@@ -3,13 +3,17 @@
 Design space (12 parameters): two patterned layers between air and a substrate
 of eps 2.25, period 300 nm in x and y, Nx = Ny = 5 (11 x 11 = 121 orders,
 242 x 242 layer matrices).  Layer j (j = 1, 2, top to bottom) holds a square
-inclusion [0, f_j P] x [0, f_j P] of material j in air:
+inclusion [0, f_j P] x [0, f_j P] of material j in air.  Two sets of ranges
+(SPACES): "sweep", the default, a design sweep around a nominal grating, and
+"wide":
 
-  f1, f2        fill fractions            U(0.2, 0.8)
-  d1, d2        thicknesses               U(50, 200) nm
-  eps1, eps2    Re eps U(2, 6), Im eps U(0, 0.1) of each material
-  lambda        wavelength                U(400, 800) nm
-  theta, phi    incidence                 theta U(0, 30) deg, phi U(0, 45) deg
+                                          sweep          wide
+  f1, f2        fill fractions            U(0.3, 0.7)    U(0.2, 0.8)
+  d1, d2        thicknesses (nm)          U(60, 140)     U(50, 200)
+  eps1, eps2    Re eps of each material   U(2, 3.5)      U(2, 6)
+                Im eps                    U(0, 0.05)     U(0, 0.1)
+  lambda        wavelength (nm)           U(400, 800)    U(400, 800)
+  theta, phi    incidence (deg)           theta U(0, 20) theta U(0, 30), phi U(0, 45)
   pol           TE (-sin phi, cos phi) or TM (cos theta cos phi, cos theta sin phi)
                 (the plane-wave amplitudes of sol/rcwa_bridge.cpp:187-196, without
                 its theta = 0 special case so that the map stays continuous)
@@ -49,10 +53,24 @@ ORDERS = [(m, n) for m in (-1, 0, 1) for n in (-1, 0, 1)]
 N_OUT = 2 * len(ORDERS)
 PARAMS = ["fill1", "fill2", "d1", "d2", "eps1_re", "eps1_im", "eps2_re", "eps2_im", "lambda",
           "theta", "phi", "pol"]
-RANGES = {"fill1": (0.2, 0.8), "fill2": (0.2, 0.8), "d1": (50e-9, 200e-9),
-          "d2": (50e-9, 200e-9), "eps1_re": (2.0, 6.0), "eps1_im": (0.0, 0.1),
-          "eps2_re": (2.0, 6.0), "eps2_im": (0.0, 0.1), "lambda": (400e-9, 800e-9),
-          "theta": (0.0, 30.0), "phi": (0.0, 45.0), "pol": (0.0, 1.0)}
+# Parameter ranges of the two design spaces (see the module docstring).
+SPACES = {
+    "sweep": {"fill1": (0.3, 0.7), "fill2": (0.3, 0.7), "d1": (60e-9, 140e-9),
+              "d2": (60e-9, 140e-9), "eps1_re": (2.0, 3.5), "eps1_im": (0.0, 0.05),
+              "eps2_re": (2.0, 3.5), "eps2_im": (0.0, 0.05), "lambda": (400e-9, 800e-9),
+              "theta": (0.0, 20.0), "phi": (0.0, 45.0), "pol": (0.0, 1.0)},
+    "wide": {"fill1": (0.2, 0.8), "fill2": (0.2, 0.8), "d1": (50e-9, 200e-9),
+             "d2": (50e-9, 200e-9), "eps1_re": (2.0, 6.0), "eps1_im": (0.0, 0.1),
+             "eps2_re": (2.0, 6.0), "eps2_im": (0.0, 0.1), "lambda": (400e-9, 800e-9),
+             "theta": (0.0, 30.0), "phi": (0.0, 45.0), "pol": (0.0, 1.0)},
+}
+DEFAULT_SPACE = "sweep"
+RANGES = SPACES[DEFAULT_SPACE]
+
+
+def ranges(space=DEFAULT_SPACE) -> dict:
+    """The parameter ranges of a design space (a name of SPACES or a dict)."""
+    return SPACES[space] if isinstance(space, str) else space
 N_IN = len(PARAMS)
 SEED_TRAIN = 20261203
 SEED_HELDOUT = 20261204
@@ -67,30 +85,32 @@ def order_index(m, n, nX=NX, nY=NY):
 ORDER_IDX = np.array([order_index(m, n) for m, n in ORDERS])
 
 
-def sample_geometries(count: int, seed: int) -> np.ndarray:
+def sample_geometries(count: int, seed: int, space=DEFAULT_SPACE) -> np.ndarray:
     """count x 11 parameter rows (all but pol) from splitmix64 uniforms."""
+    rg = ranges(space)
     u = SplitMix64(seed).uniforms(count * (N_IN - 1)).reshape(count, N_IN - 1)
-    lo = np.array([RANGES[k][0] for k in PARAMS[:-1]])
-    hi = np.array([RANGES[k][1] for k in PARAMS[:-1]])
+    lo = np.array([rg[k][0] for k in PARAMS[:-1]])
+    hi = np.array([rg[k][1] for k in PARAMS[:-1]])
     return lo + (hi - lo) * u
 
 
-def sample_queries(count: int, seed: int) -> np.ndarray:
+def sample_queries(count: int, seed: int, space=DEFAULT_SPACE) -> np.ndarray:
     """count x 12 parameter rows with a random polarisation (0 = TE, 1 = TM)."""
-    g = sample_geometries(count, seed)
+    g = sample_geometries(count, seed, space)
     pol = (SplitMix64(seed ^ 0x5DEECE66D).uniforms(count) >= 0.5).astype(np.float64)
     return np.concatenate([g, pol[:, None]], axis=1)
 
 
-def scale_inputs(p: np.ndarray) -> np.ndarray:
-    """Parameters -> network inputs in [-1, 1] (float32)."""
+def scale_inputs(p: np.ndarray, space=DEFAULT_SPACE) -> np.ndarray:
+    """Parameters -> network inputs in [-1, 1] over the space's ranges (float32)."""
     p = np.asarray(p, dtype=np.float64)
-    lo = np.array([RANGES[k][0] for k in PARAMS])
-    hi = np.array([RANGES[k][1] for k in PARAMS])
+    rg = ranges(space)
+    lo = np.array([rg[k][0] for k in PARAMS])
+    hi = np.array([rg[k][1] for k in PARAMS])
     return (2.0 * (p - lo) / (hi - lo) - 1.0).astype(np.float32)
 
 
-def features(p: np.ndarray) -> np.ndarray:
+def features(p: np.ndarray, space=DEFAULT_SPACE) -> np.ndarray:
     """Network inputs of parameter rows: the 12 scaled parameters, then for each
     of the 8 non-zero orders of ORDERS in air and in the substrate the
     normalised z wavenumber kz / k0 = sqrt(n^2 - (sin t cos f + m lam / P)^2 -
@@ -99,7 +119,7 @@ def features(p: np.ndarray) -> np.ndarray:
     order's power switches on where its kz crosses zero (the Rayleigh
     anomalies), so giving it to the network keeps the map it learns smooth."""
     p = np.asarray(p, dtype=np.float64)
-    x = scale_inputs(p)
+    x = scale_inputs(p, space)
     r = p[:, 8] / PERIOD
     th, ph = np.deg2rad(p[:, 9]), np.deg2rad(p[:, 10])
     kx, ky = np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph)
@@ -185,10 +205,10 @@ def solve_queries(ctx: Context, queries: np.ndarray, chunk: int = 4096):
 
 
 def make_dataset(ctx: Context, n_solves: int, seed: int = SEED_TRAIN, chunk: int = 4096,
-                 progress=None):
+                 progress=None, space=DEFAULT_SPACE):
     """GPU-generated training rows: every solved geometry gives a TE and a TM
     row.  Returns (params (2 n, 12), targets (2 n, 18) float64, status (n,))."""
-    g = sample_geometries(n_solves, seed)
+    g = sample_geometries(n_solves, seed, space)
     res, status = solve_geometries(ctx, g, chunk, progress)
     params = np.concatenate([np.concatenate([g, np.zeros((n_solves, 1))], 1),
                              np.concatenate([g, np.ones((n_solves, 1))], 1)])
@@ -295,45 +315,55 @@ class MLP:
 class Surrogate:
     """Network + calibrated error estimate + plain-solver fallback."""
 
-    def __init__(self, ctx: Context, members=4, width=512, depth=4, seed=1, tol=1e-3):
+    def __init__(self, ctx: Context, members=4, width=512, depth=4, seed=1, tol=1e-3,
+                 space=DEFAULT_SPACE):
         self.ctx = ctx
+        self.space = space
         self.net = MLP(ctx, N_FEAT, width, depth, N_OUT, members, seed)
         self.tol = float(tol)
         self.kappa = None
+        self.tau = None
 
     def fit(self, params, targets, epochs, batch=4096, lr_max=1e-3, lr_min=1e-5, seed=7,
             use_graph=True):
-        self.net.set_data(features(params), targets)
+        self.net.set_data(features(params, self.space), targets)
         return self.net.train(epochs, batch, lr_max, lr_min, seed, use_graph)
 
-    def calibrate(self, params, targets, margin=1.5):
-        """Scales of the error estimate, one per output: kappa_c is the smallest
-        scale for which every calibration query whose output c is off by more
-        than tol has kappa_c x spread_c > tol, times margin (the spread of an
-        output is max - min over the members).  The estimate of a query is
-        max_c kappa_c spread_c.  Returns (kappa, fallback fraction on this set)."""
-        x = features(params)
+    def calibrate(self, params, targets, margin=1.5, tol=None):
+        """Threshold of the error estimate for tolerance tol (default: the
+        surrogate's).  The estimate of a query is its largest member spread,
+        max_c (max - min over members of output c); tau is the smallest
+        estimate among the calibration queries whose answer is off by more than
+        tol in some output, divided by margin, so every such calibration query
+        is flagged.  (One scale for all outputs flags far fewer queries than a
+        scale per output calibrated the same way: a query off in one output is
+        often caught by the spread of another.)  The scale tol / tau goes to the
+        network (orcwa_mlp_set_error_scale), whose estimate is then compared
+        with tol.  Returns (tau, fallback fraction on this set)."""
+        if tol is not None:
+            self.tol = float(tol)
+        x = features(params, self.space)
         y = self.net.forward(x).astype(np.float64)             # (members, rows, n_out)
         mean = y.mean(0)
-        spread = y.max(0) - y.min(0)
-        err = np.abs(mean - targets)
-        kappa = np.ones(N_OUT)
-        for c in range(N_OUT):
-            bad = err[:, c] > self.tol
-            if bad.any():
-                kappa[c] = margin * float(np.max(self.tol / np.maximum(spread[bad, c], 1e-30)))
-        self.kappa = kappa
-        self.net.set_error_scale(kappa)
+        spread = (y.max(0) - y.min(0)).max(1)
+        bad = (np.abs(mean - targets).max(1) > self.tol) & ~self._phys(mean)
+        tau = float(spread[bad].min()) / margin if bad.any() else float(spread.max())
+        self.tau = max(tau, 1e-30)
+        self.kappa = np.full(N_OUT, self.tol / self.tau)
+        self.net.set_error_scale(self.kappa)
         m, est = self.net.predict(x)
         flagged = self._flags(m, est)
-        return kappa, float(flagged.mean())
+        return self.tau, float(flagged.mean())
 
-    def _flags(self, mean, spread):
+    def _phys(self, mean):
         n = len(ORDERS)
         tot = mean[:, :n].sum(1) + mean[:, n:].sum(1)
-        # spread: the scaled estimate max_c kappa_c (max - min)_c of predict()
-        return ((spread > self.tol) | (mean.min(1) < -self.tol) |
-                (mean.max(1) > 1.0 + self.tol) | (tot > 1.0 + self.tol))
+        return ((mean.min(1) < -self.tol) | (mean.max(1) > 1.0 + self.tol) |
+                (tot > 1.0 + self.tol))
+
+    def _flags(self, mean, est):
+        # est: the scaled estimate (tol / tau) max_c spread_c of predict()
+        return (est > self.tol) | self._phys(mean)
 
     def query(self, params):
         """Answer 12-parameter queries: network prediction, with the flagged ones
@@ -343,7 +373,7 @@ class Surrogate:
             raise RuntimeError("calibrate() the error estimate before querying")
         p = np.asarray(params, dtype=np.float64)
         t0 = time.perf_counter()
-        mean, spread = self.net.predict(features(p))
+        mean, spread = self.net.predict(features(p, self.space))
         t1 = time.perf_counter()
         flagged = self._flags(mean, spread)
         out = mean.astype(np.float64)

@@ -134,7 +134,7 @@ def test_surrogate_with_fallback(gpu_ctx, surrogate_data):
     try:
         loss = sur.fit(params, targets, epochs=40, batch=1024, lr_max=2e-3, lr_min=2e-5)
         assert loss[-1] < 0.1 * loss[0]
-        kappa, cal_frac = sur.calibrate(cal, cal_t)
+        tau, cal_frac = sur.calibrate(cal, cal_t)
         q = sg.sample_queries(2048, sg.SEED_HELDOUT)
         t0 = time.perf_counter()
         plain, _ = sg.solve_queries(gpu_ctx, q)
@@ -143,7 +143,7 @@ def test_surrogate_with_fallback(gpu_ctx, surrogate_data):
         err = np.abs(ans - plain).max()
         net_err = np.abs(sur.net.predict(sg.features(q))[0] - plain).max(1)
         print(f"\n[surrogate] data {gen_s:.1f} s, train loss {loss[0]:.2e} -> {loss[-1]:.2e}, "
-              f"kappa max {kappa.max():.3g}, fallback {flagged.mean():.3f} (calibration {cal_frac:.3f}), "
+              f"spread threshold {tau:.3g}, fallback {flagged.mean():.3f} (calibration {cal_frac:.3f}), "
               f"network-only max err {net_err.max():.2e} median {np.median(net_err):.2e}, "
               f"after fallback {err:.2e}; plain {plain_s:.2f} s, surrogate {tm['total_s']:.2f} s "
               f"(network {tm['network_s'] * 1e3:.1f} ms)", flush=True)

@@ -128,3 +128,10 @@ def test_vectorised_splitmix_equals_the_scalar_sequence():
     seq = np.array([a.uniform() for _ in range(500)])
     assert np.array_equal(seq, np.concatenate([b.uniforms(123), b.uniforms(377)]))
     assert a.state == b.state
+
+
+def test_features_layout():
+    q = sg.sample_queries(64, 3)
+    f = sg.features(q)
+    assert f.shape == (64, sg.N_FEAT) and f.dtype == np.float32
+    assert np.array_equal(f[:, :sg.N_IN], sg.scale_inputs(q))

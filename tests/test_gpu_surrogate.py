@@ -135,9 +135,9 @@ def test_surrogate_with_fallback(gpu_ctx, surrogate_data):
         loss = sur.fit(params, targets, epochs=40, batch=1024, lr_max=2e-3, lr_min=2e-5)
         assert loss[-1] < 0.1 * loss[0]
         tau, cal_frac = sur.calibrate(cal, cal_t)
-        q = sg.sample_queries(2048, sg.SEED_HELDOUT)
+        q = sg.sample_queries(65536, sg.SEED_HELDOUT)
         t0 = time.perf_counter()
-        plain, _ = sg.solve_queries(gpu_ctx, q)
+        plain, _ = sg.solve_queries(gpu_ctx, q, chunk=8192)
         plain_s = time.perf_counter() - t0
         ans, flagged, tm = sur.query(q)
         err = np.abs(ans - plain).max()

@@ -863,16 +863,19 @@ __global__ void k_laswp(int n, int ncols_total, int k0, int nb, int c0, int c1, 
 constexpr int LASWP_MAXS = 64, LASWP_NC = 16;
 __global__ __launch_bounds__(256) void k_laswp_panel(int n, int k0, int nb, int c0, int c1,
                                                      cd* const* Ap, const int* ipiv) {
-  __shared__ int R[LASWP_MAXS], src[LASWP_MAXS];
+  __shared__ int R[LASWP_MAXS], src[LASWP_MAXS], piv[LASWP_MAXS];
   __shared__ int s_ns;
   __shared__ cd tile[LASWP_MAXS][LASWP_NC];
   const int b = blockIdx.y;
-  const int* pv = ipiv + (long long)b * n;
+  // the panel's pivots, loaded together (thread 0's composition below would
+  // otherwise wait on one global load per interchange)
+  if (threadIdx.x < nb) piv[threadIdx.x] = ipiv[(long long)b * n + k0 + threadIdx.x];
+  __syncthreads();
   if (threadIdx.x == 0) {
     int ns = nb;
     for (int j = 0; j < nb; ++j) { R[j] = k0 + j; src[j] = j; }
     for (int j = 0; j < nb; ++j) {
-      const int p = pv[k0 + j];
+      const int p = piv[j];
       int sp = -1;
       if (p < k0 + nb) {
         sp = p - k0;
@@ -946,25 +949,43 @@ __global__ __launch_bounds__(256) void k_apply_perm(int n, int nrhs, int ncw, co
 }
 
 // X <- L11^{-1} X for the nb x ncols block of rows [k0, k0+nb), columns
-// [c0, c0+ncols) of X (ld = n), L11 unit lower at (k0,k0) of L (ld = n).
+// [c0, c0+ncols) of X (ld = n), L11 unit lower at (k0,k0) of L (ld = n); with
+// UNIT_LOWER false, U11^{-1} X for the upper, non-unit block.  One column per
+// thread; the block moves between HBM and registers through LDS in slabs of
+// TR_SLAB rows, so that consecutive threads touch consecutive rows of a
+// column (a column's 32-row segment is 512 contiguous bytes; read directly,
+// the lanes of a wave would each touch another column).
+constexpr int TR_SLAB = 8;
 template <bool UNIT_LOWER>
 __global__ __launch_bounds__(256) void k_trsm_block(int n, int k0, int nb, int c0, int ncols,
                                                     const cd* const* Lp, cd* const* Xp) {
   __shared__ cd sT[NB][NB];
+  __shared__ cd sX[256][TR_SLAB + 1];
   const int b = blockIdx.y;
   const cd* __restrict__ Lm = Lp[b];
   cd* __restrict__ X = Xp[b];
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nb * nb; e += 256) {
     const int i = e % nb, j = e / nb;
     sT[i][j] = Lm[(k0 + i) + (long long)(k0 + j) * n];
   }
-  __syncthreads();
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncols) return;
-  cd* __restrict__ x = X + (long long)(c0 + col) * n + k0;
+  const int cb = blockIdx.x * 256;   // this workgroup's first column (from c0)
   cd r[NB];
 #pragma unroll
-  for (int i = 0; i < NB; ++i) r[i] = (i < nb) ? x[i] : mk(0.0, 0.0);
+  for (int q = 0; q < NB / TR_SLAB; ++q) {
+#pragma unroll
+    for (int i = 0; i < TR_SLAB; ++i) {
+      const int e = tid + 256 * i, cc = e / TR_SLAB, rr = e % TR_SLAB;
+      const int row = TR_SLAB * q + rr;
+      sX[cc][rr] = (cb + cc < ncols && row < nb) ? X[(long long)(c0 + cb + cc) * n + k0 + row]
+                                                 : mk(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < TR_SLAB; ++rr) r[TR_SLAB * q + rr] = sX[tid][rr];
+    __syncthreads();
+  }
+  // (columns past ncols hold zeros and are not stored)
   if (UNIT_LOWER) {
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -987,8 +1008,18 @@ __global__ __launch_bounds__(256) void k_trsm_block(int n, int k0, int nb, int c
     }
   }
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < nb) x[i] = r[i];
+  for (int q = 0; q < NB / TR_SLAB; ++q) {
+#pragma unroll
+    for (int rr = 0; rr < TR_SLAB; ++rr) sX[tid][rr] = r[TR_SLAB * q + rr];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TR_SLAB; ++i) {
+      const int e = tid + 256 * i, cc = e / TR_SLAB, rr = e % TR_SLAB;
+      const int row = TR_SLAB * q + rr;
+      if (cb + cc < ncols && row < nb) X[(long long)(c0 + cb + cc) * n + k0 + row] = sX[cc][rr];
+    }
+    __syncthreads();
+  }
 }
 
 void zgetrf_batched(hipStream_t s, int n, cd* const* A, int* ipiv, int* info,

@@ -863,19 +863,16 @@ __global__ void k_laswp(int n, int ncols_total, int k0, int nb, int c0, int c1, 
 constexpr int LASWP_MAXS = 64, LASWP_NC = 16;
 __global__ __launch_bounds__(256) void k_laswp_panel(int n, int k0, int nb, int c0, int c1,
                                                      cd* const* Ap, const int* ipiv) {
-  __shared__ int R[LASWP_MAXS], src[LASWP_MAXS], piv[LASWP_MAXS];
+  __shared__ int R[LASWP_MAXS], src[LASWP_MAXS];
   __shared__ int s_ns;
   __shared__ cd tile[LASWP_MAXS][LASWP_NC];
   const int b = blockIdx.y;
-  // the panel's pivots, loaded together (thread 0's composition below would
-  // otherwise wait on one global load per interchange)
-  if (threadIdx.x < nb) piv[threadIdx.x] = ipiv[(long long)b * n + k0 + threadIdx.x];
-  __syncthreads();
+  const int* pv = ipiv + (long long)b * n;
   if (threadIdx.x == 0) {
     int ns = nb;
     for (int j = 0; j < nb; ++j) { R[j] = k0 + j; src[j] = j; }
     for (int j = 0; j < nb; ++j) {
-      const int p = piv[j];
+      const int p = pv[k0 + j];
       int sp = -1;
       if (p < k0 + nb) {
         sp = p - k0;
@@ -928,7 +925,7 @@ __global__ __launch_bounds__(PERM_MB) void k_ipiv_to_perm(int n, const int* ipiv
 }
 
 // B <- P^T B as a gather through LDS: ncw (<= PERM_NC) columns per workgroup
-constexpr int PERM_NC = 8;
+constexpr int PERM_NC = 4;
 __global__ __launch_bounds__(256) void k_apply_perm(int n, int nrhs, int ncw, const int* perm,
                                                     cd* const* Bp) {
   extern __shared__ cd X[];   // [ncw][n]
@@ -965,9 +962,10 @@ __global__ __launch_bounds__(256) void k_trsm_block(int n, int k0, int nb, int c
   const cd* __restrict__ Lm = Lp[b];
   cd* __restrict__ X = Xp[b];
   const int tid = threadIdx.x;
-  for (int e = tid; e < nb * nb; e += 256) {
-    const int i = e % nb, j = e / nb;
-    sT[i][j] = Lm[(k0 + i) + (long long)(k0 + j) * n];
+  // (zero outside nb x nb, so the unrolled products below need no guard)
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    sT[i][j] = (i < nb && j < nb) ? Lm[(k0 + i) + (long long)(k0 + j) * n] : mk(0.0, 0.0);
   }
   const int cb = blockIdx.x * 256;   // this workgroup's first column (from c0)
   cd r[NB];
@@ -1000,8 +998,7 @@ __global__ __launch_bounds__(256) void k_trsm_block(int n, int k0, int nb, int c
     for (int ii = NB - 1; ii >= 0; --ii) {
       if (ii < nb) {
 #pragma unroll
-        for (int k = 0; k < NB; ++k)
-          if (k > ii && k < nb) r[ii] -= sT[ii][k] * r[k];
+        for (int k = ii + 1; k < NB; ++k) r[ii] -= sT[ii][k] * r[k];
         cd d = sT[ii][ii];
         if (d.re != 0.0 || d.im != 0.0) r[ii] = cdiv(r[ii], d);
       }

@@ -62,13 +62,11 @@ def test_zgemm_ops(gpu_ctx, tA, cA, tB, cB):
                                           (97, 40, 3), (450, 900, 3), (513, 9, 2),
                                           (600, 20, 3), (1300, 11, 2), (2100, 5, 2)])
 def test_zgesv(gpu_ctx, n, nrhs, batch):
-    # even batches solve through the row-permutation workspace: up to n = 512
-    # one launch per 16-column panel (the panel in MFMA registers, 4 or 8
-    # waves), above it a gather and the blocked triangular solves; odd
-    # batches swap by swap and the blocked solves.  Panels of more than 512
-    # rows factor in global memory, shorter ones in registers; the gather
-    # takes fewer columns per workgroup as n grows (7 at n = 513, 3 at 1300,
-    # 1 at 2100)
+    # even batches apply the row interchanges of the solve as one gather,
+    # odd ones swap by swap (both paths of zgetrs_batched); panels of more
+    # than 512 rows factor in global memory, shorter ones in registers; the
+    # gather takes fewer columns per workgroup as n grows (4 at n = 513, 2 at
+    # 1300, 1 at 2100)
     A, B = crand(batch, n, n), crand(batch, n, nrhs)
     X, info = g.zgesv(gpu_ctx, A, B)
     assert not info.any()

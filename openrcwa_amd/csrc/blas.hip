@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
     int M, int N, int K, cd alpha, cd beta,
     const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
     const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
-    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
+    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC, int kup) {
   using TS = TileShape<TM, TN>;
   constexpr int BI = TS::BI, BJ = TS::BJ, LA = TS::LA, LB = TS::LB, WC = TS::WC;
   // rows padded by one double: the k-fast stores (16 lanes down one column)
@@ -96,6 +96,9 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(
   const cd* __restrict__ B = Bp[b] + offB;
   cd* __restrict__ C = Cp[b] + offC;
   const int m0 = bx * TM, n0 = by * TN;
+  // op(B) upper triangular from diagonal kup on: its rows past this tile's
+  // last column + kup are zero, so the K range ends there
+  if (kup >= 0) K = min(K, n0 + TN + kup);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave / WC) * (16 * BI), wn = (wave % WC) * (16 * BJ);
   const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_f32(
     int M, int N, int K, cd alpha, cd beta,
     const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
     const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
-    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
+    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC, int kup) {
   constexpr int KS = 16;
   constexpr int WR = (TM >= 32 && TN >= 32) ? 2 : (TN == 16 ? 4 : 1), WC = 4 / WR;
   constexpr int BI = TM / (16 * WR), BJ = TN / (16 * WC);
@@ -256,6 +259,9 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_f32(
   const cd* __restrict__ B = Bp[b] + offB;
   cd* __restrict__ C = Cp[b] + offC;
   const int m0 = bx * TM, n0 = by * TN;
+  // op(B) upper triangular from diagonal kup on: its rows past this tile's
+  // last column + kup are zero, so the K range ends there
+  if (kup >= 0) K = min(K, n0 + TN + kup);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave / WC) * (16 * BI), wn = (wave % WC) * (16 * BJ);
   const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
@@ -405,7 +411,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_bf16(
     int M, int N, int K, cd alpha, cd beta,
     const cd* const* __restrict__ Ap, long long offA, long long rsA, long long csA, int conjA,
     const cd* const* __restrict__ Bp, long long offB, long long rsB, long long csB, int conjB,
-    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC) {
+    cd* const* __restrict__ Cp, long long offC, long long rsC, long long csC, int kup) {
   constexpr int KS = 32, LDK = KS + 8;        // k-fastest rows, padded by 8 (16 bytes)
   constexpr int WR = (TM >= 32 && TN >= 32) ? 2 : (TN == 16 ? 4 : 1), WC = 4 / WR;
   constexpr int BI = TM / (16 * WR), BJ = TN / (16 * WC);
@@ -419,6 +425,9 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_bf16(
   const cd* __restrict__ B = Bp[b] + offB;
   cd* __restrict__ C = Cp[b] + offC;
   const int m0 = bx * TM, n0 = by * TN;
+  // op(B) upper triangular from diagonal kup on: its rows past this tile's
+  // last column + kup are zero, so the K range ends there
+  if (kup >= 0) K = min(K, n0 + TN + kup);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave / WC) * (16 * BI), wn = (wave % WC) * (16 * BJ);
   const double sa = conjA ? -1.0 : 1.0, sb = conjB ? -1.0 : 1.0;
@@ -532,13 +541,13 @@ static void zgemm_launch(hipStream_t s, int M, int N, int K, cd alpha, const cd*
                          long long offA, long long rsA, long long csA, int conjA,
                          const cd* const* B, long long offB, long long rsB, long long csB,
                          int conjB, cd beta, cd* const* C, long long offC, long long rsC,
-                         long long csC, int batch, int prec) {
+                         long long csC, int batch, int prec, int kup) {
   if (M <= 0 || N <= 0) return;
   dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, batch);
   const bool b0 = (beta.re == 0.0 && beta.im == 0.0);
 #define ORCWA_GEMM_ARGS                                                                        \
   grid, dim3(256), 0, s, M, N, K, alpha, beta, A, offA, rsA, csA, conjA, B, offB, rsB, csB,   \
-      conjB, C, offC, rsC, csC
+      conjB, C, offC, rsC, csC, kup
   switch (prec) {
     case GEMM_FP32:
       if (b0) hipLaunchKernelGGL((k_zgemm_f32<true, TM, TN>), ORCWA_GEMM_ARGS);
@@ -570,31 +579,31 @@ static void zgemm_rows(hipStream_t s, int M, int N, int K, cd alpha, const cd* c
                        long long offA, long long rsA, long long csA, int conjA, const cd* const* B,
                        long long offB, long long rsB, long long csB, int conjB, cd beta,
                        cd* const* C, long long offC, long long rsC, long long csC, int batch,
-                       int prec) {
+                       int prec, int kup) {
   // (a 4-wave tile needs at least four 16 x 16 blocks)
   if constexpr (TN == 64) {
     if (M <= 16) {
       zgemm_launch<16, 64>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                           beta, C, offC, rsC, csC, batch, prec);
+                           beta, C, offC, rsC, csC, batch, prec, kup);
       return;
     }
   }
   if constexpr (TN >= 32) {
     if (M <= 32) {
       zgemm_launch<32, TN>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                           beta, C, offC, rsC, csC, batch, prec);
+                           beta, C, offC, rsC, csC, batch, prec, kup);
       return;
     }
   }
   zgemm_launch<64, TN>(s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB, conjB,
-                       beta, C, offC, rsC, csC, batch, prec);
+                       beta, C, offC, rsC, csC, batch, prec, kup);
 }
 
 void zgemm_batched_prec(int prec, hipStream_t s, int M, int N, int K, cd alpha,
                         const cd* const* A, long long offA, long long rsA, long long csA,
                         int conjA, const cd* const* B, long long offB, long long rsB,
                         long long csB, int conjB, cd beta, cd* const* C, long long offC,
-                        long long rsC, long long csC, int batch) {
+                        long long rsC, long long csC, int batch, int kup) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   // 64 x 64 tiles over the largest multiple-of-64 block, narrower tiles on the
   // remaining row and column strips (n = 450: 448 + a 2-wide strip, instead
@@ -605,28 +614,29 @@ void zgemm_batched_prec(int prec, hipStream_t s, int M, int N, int K, cd alpha,
     // current plan (64 x 64 unless set)
     if (g_gemm_tile == GEMM_TILE_32x64)
       zgemm_launch<32, 64>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
-                           conjB, beta, C, offC, rsC, csC, batch, prec);
+                           conjB, beta, C, offC, rsC, csC, batch, prec, kup);
     else if (g_gemm_tile == GEMM_TILE_64x32)
       zgemm_launch<64, 32>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
-                           conjB, beta, C, offC, rsC, csC, batch, prec);
+                           conjB, beta, C, offC, rsC, csC, batch, prec, kup);
     else
       zgemm_launch<64, 64>(s, M0, N0, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
-                           conjB, beta, C, offC, rsC, csC, batch, prec);
+                           conjB, beta, C, offC, rsC, csC, batch, prec, kup);
   }
   if (mr > 0 && N0 > 0)     // bottom strip
     zgemm_rows<64>(s, mr, N0, K, alpha, A, offA + M0 * rsA, rsA, csA, conjA, B, offB, rsB, csB,
-                   conjB, beta, C, offC + M0 * rsC, rsC, csC, batch, prec);
+                   conjB, beta, C, offC + M0 * rsC, rsC, csC, batch, prec, kup);
   if (nr > 0) {             // right strip, all rows (corner included)
     const long long oB = offB + N0 * csB, oC = offC + N0 * csC;
+    const int kr = kup >= 0 ? kup + N0 : -1;   // the strip's column 0 is column N0
     if (nr <= 16)
       zgemm_rows<16>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
-                     oC, rsC, csC, batch, prec);
+                     oC, rsC, csC, batch, prec, kr);
     else if (nr <= 32)
       zgemm_rows<32>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
-                     oC, rsC, csC, batch, prec);
+                     oC, rsC, csC, batch, prec, kr);
     else
       zgemm_rows<64>(s, M, nr, K, alpha, A, offA, rsA, csA, conjA, B, oB, rsB, csB, conjB, beta, C,
-                     oC, rsC, csC, batch, prec);
+                     oC, rsC, csC, batch, prec, kr);
   }
 }
 
@@ -637,6 +647,15 @@ void zgemm_batched(hipStream_t s, int M, int N, int K, cd alpha,
                    int batch) {
   zgemm_batched_prec(GEMM_FP64, s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
                      conjB, beta, C, offC, rsC, csC, batch);
+}
+
+void zgemm_batched_upper_b(hipStream_t s, int M, int N, int K, cd alpha,
+                           const cd* const* A, long long offA, long long rsA, long long csA,
+                           int conjA, const cd* const* B, long long offB, long long rsB,
+                           long long csB, int conjB, cd beta, cd* const* C, long long offC,
+                           long long rsC, long long csC, int batch) {
+  zgemm_batched_prec(GEMM_FP64, s, M, N, K, alpha, A, offA, rsA, csA, conjA, B, offB, rsB, csB,
+                     conjB, beta, C, offC, rsC, csC, batch, 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -623,11 +623,13 @@ static void trevc_blocked(hipStream_t s, int n, const cd* const* T, cd* const* X
   const int nblk = (n + TB - 1) / TB;
   for (int bk = nblk - 1; bk >= 0; --bk) {
     const int i0 = bk * TB, i1 = std::min(n, i0 + TB);
+    // (X(i1:n, i1:n) is upper triangular: the identity's zeros below the
+    // diagonal stay untouched)
     if (i1 < n)
-      zgemm_batched(s, i1 - i0, n - i1, n - i1, mk(-1.0, 0.0), T,
-                    (long long)i0 + (long long)i1 * n, 1, n, 0, (const cd* const*)X,
-                    (long long)i1 + (long long)i1 * n, 1, n, 0, mk(0.0, 0.0), X,
-                    (long long)i0 + (long long)i1 * n, 1, n, batch);
+      zgemm_batched_upper_b(s, i1 - i0, n - i1, n - i1, mk(-1.0, 0.0), T,
+                            (long long)i0 + (long long)i1 * n, 1, n, 0, (const cd* const*)X,
+                            (long long)i1 + (long long)i1 * n, 1, n, 0, mk(0.0, 0.0), X,
+                            (long long)i0 + (long long)i1 * n, 1, n, batch);
     hipLaunchKernelGGL(k_trevc_block, dim3((n - i0 + 255) / 256, batch), dim3(256), 0, s, n, i0,
                        i1, T, X);
   }
@@ -815,8 +817,9 @@ void zgeev_batched(hipStream_t s, int n, cd* const* A, const cd* const* Aorig, c
     trevc_blocked(s, n, Ac, ws.X, batch);
   else
     hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(256), 0, s, n, Ac, ws.X);
-  zgemm_batched(s, n, n, n, mk(1.0, 0.0), (const cd* const*)ws.Z, 0, 1, n, 0,
-                (const cd* const*)ws.X, 0, 1, n, 0, mk(0.0, 0.0), W, 0, 1, n, batch);
+  // W = Z X, X upper triangular (the eigenvectors of T)
+  zgemm_batched_upper_b(s, n, n, n, mk(1.0, 0.0), (const cd* const*)ws.Z, 0, 1, n, 0,
+                        (const cd* const*)ws.X, 0, 1, n, 0, mk(0.0, 0.0), W, 0, 1, n, batch);
   hipLaunchKernelGGL(k_eig_finish, dim3(batch), dim3(256), 0, s, n, Ac, W, lam);
   if (Aorig) {
     zgemm_batched(s, n, n, n, mk(1.0, 0.0), Aorig, 0, 1, n, 0, (const cd* const*)W, 0, 1, n, 0,

@@ -34,7 +34,14 @@ void zgemm_batched_prec(int prec, hipStream_t s, int M, int N, int K, cd alpha,
                         const cd* const* A, long long offA, long long rsA, long long csA,
                         int conjA, const cd* const* B, long long offB, long long rsB,
                         long long csB, int conjB, cd beta, cd* const* C, long long offC,
-                        long long rsC, long long csC, int batch);
+                        long long rsC, long long csC, int batch, int kup = -1);
+// zgemm_batched for an upper-triangular op(B) (op(B)[k][j] == 0 for k > j,
+// stored zeros included): each output tile's K range ends at its last column.
+void zgemm_batched_upper_b(hipStream_t s, int M, int N, int K, cd alpha,
+                           const cd* const* A, long long offA, long long rsA, long long csA,
+                           int conjA, const cd* const* B, long long offB, long long rsB,
+                           long long csB, int conjB, cd beta, cd* const* C, long long offC,
+                           long long rsC, long long csC, int batch);
 
 // Tile of the multiple-of-64 main block of the GEMMs the calling thread issues
 // (the other tiles cover the edge strips); set per plan by the autotuner.

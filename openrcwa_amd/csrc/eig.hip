@@ -165,7 +165,9 @@ __global__ __launch_bounds__(256) void k_hess_panel(int n, int i0, int nb, cd* c
   extern __shared__ cd sm[];
   cd* sb = sm;          // current column, rows i0+1..n-1
   cd* sv = sm + n;      // current reflector
-  __shared__ cd Ts[HB][HB + 1];
+  // T of the panel's block reflector, upper triangular, packed by columns:
+  // T(r, c) at c (c + 1) / 2 + r, r <= c
+  __shared__ cd Ts[HB * (HB + 1) / 2];
   __shared__ cd s_w[HB], s_t[HB], s_row[HB];
   __shared__ double sred[4];
   __shared__ cd s_tau, s_scale;
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(256) void k_hess_panel(int n, int i0, int nb, cd* c
       __syncthreads();
       if (tid < j) {
         cd acc = mk(0.0, 0.0);
-        for (int q = 0; q <= tid; ++q) acc += conj(Ts[q][tid]) * s_w[q];
+        for (int q = 0; q <= tid; ++q) acc += conj(Ts[tid * (tid + 1) / 2 + q]) * s_w[q];
         s_t[tid] = acc;
       }
       __syncthreads();
@@ -296,15 +298,15 @@ __global__ __launch_bounds__(256) void k_hess_panel(int n, int i0, int nb, cd* c
     // T(0:j, j) = -tau T(0:j, 0:j) t,  T(j, j) = tau
     if (tid < j) {
       cd acc = mk(0.0, 0.0);
-      for (int q = tid; q < j; ++q) acc += Ts[tid][q] * s_t[q];
-      Ts[tid][j] = -(t * acc);
+      for (int q = tid; q < j; ++q) acc += Ts[q * (q + 1) / 2 + tid] * s_t[q];
+      Ts[j * (j + 1) / 2 + tid] = -(t * acc);
     }
-    if (tid == j) Ts[j][j] = t;
+    if (tid == j) Ts[j * (j + 1) / 2 + j] = t;
     __syncthreads();
   }
   for (int e = tid; e < HB * HB; e += 256) {
     const int r = e % HB, q = e / HB;
-    Tg[e] = (r <= q && q < nb) ? Ts[r][q] : mk(0.0, 0.0);
+    Tg[e] = (r <= q && q < nb) ? Ts[q * (q + 1) / 2 + r] : mk(0.0, 0.0);
   }
   // columns outside the panel of V stay zero for the GEMMs
   for (long long e = (long long)nb * n + tid; e < (long long)HB * n; e += 256) V[e] = mk(0.0, 0.0);

@@ -243,9 +243,24 @@ def bench_surrogate(args):
           file=sys.stderr, flush=True)
     sur = sg.Surrogate(ctx, members=4, width=512, depth=4, seed=21)
     t0 = time.perf_counter()
-    loss = sur.fit(P, T, epochs=args.sur_epochs, batch=4096, lr_max=2e-3, lr_min=1e-5)
+    # --sur-segments k: k cosine schedules of epochs / k each, the peak
+    # learning rate halved from one to the next (warm restarts)
+    segs = max(1, args.sur_segments)
+    per = max(1, args.sur_epochs // segs)
+    losses, train_gpu_ms = [], 0.0
+    for k in range(segs):
+        lr = 2e-3 / 2 ** k
+        if k == 0:
+            losses.append(sur.fit(P, T, epochs=per, batch=4096, lr_max=lr, lr_min=1e-5))
+        else:
+            losses.append(sur.net.train(per, batch=4096, lr_max=lr, lr_min=1e-5, seed=7 + k))
+        train_gpu_ms += sur.net.last_ms()
+        if segs > 1:
+            print(f"[bench] training segment {k + 1}/{segs}: {per} epochs, peak lr {lr:.1e}, "
+                  f"loss {losses[-1][-1]:.2e}, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+    loss = np.concatenate(losses)
     train_s = time.perf_counter() - t0
-    train_gpu_ms = sur.net.last_ms()
     print(f"[bench] training: {args.sur_epochs} epochs in {train_s:.1f} s, loss {loss[0]:.2e} -> "
           f"{loss[-1]:.2e}", file=sys.stderr, flush=True)
     cal = sg.sample_queries(16384, sg.SEED_CALIB)
@@ -315,10 +330,11 @@ def bench_surrogate(args):
                          "status_nonzero": int((pst != 0).sum())},
            "speedup_vs_plain": value / plain_rate,
            "data_generation": {"solves": n_solves, "s": gen_s, "solves_per_s": n_solves / gen_s},
-           "training": {"epochs": args.sur_epochs, "s": train_s, "gpu_ms": train_gpu_ms,
+           "training": {"epochs": per * segs, "segments": segs, "s": train_s,
+                        "gpu_ms": train_gpu_ms,
                         "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
                         "tflops": 6.0 * sur.net.P * sur.net.members * 2 * n_solves *
-                        args.sur_epochs / (train_gpu_ms * 1e-3) / 1e12}}
+                        per * segs / (train_gpu_ms * 1e-3) / 1e12}}
     sur.net.close()
     print(json.dumps(out), flush=True)
 
@@ -344,6 +360,9 @@ def main():
                     help="surrogate workload: solves of training data (two rows each)")
     ap.add_argument("--sur-epochs", type=int, default=150,
                     help="surrogate workload: training epochs")
+    ap.add_argument("--sur-segments", type=int, default=1,
+                    help="surrogate workload: training in this many cosine schedules (warm "
+                         "restarts), a progress line after each")
     ap.add_argument("--workload", choices=["pillar2d", "large2d", "uneven", "surrogate"],
                     default="pillar2d",
                     help="pillar2d: the metric's 15x15-order batch; large2d: 31x31 orders, "

@@ -244,12 +244,12 @@ def bench_surrogate(args):
     sur = sg.Surrogate(ctx, members=4, width=512, depth=4, seed=21)
     t0 = time.perf_counter()
     # --sur-segments k: k cosine schedules of epochs / k each, the peak
-    # learning rate halved from one to the next (warm restarts)
+    # learning rate scaled by --sur-lr-decay from one to the next (warm restarts)
     segs = max(1, args.sur_segments)
     per = max(1, args.sur_epochs // segs)
     losses, train_gpu_ms = [], 0.0
     for k in range(segs):
-        lr = 2e-3 / 2 ** k
+        lr = 2e-3 * args.sur_lr_decay ** k
         if k == 0:
             losses.append(sur.fit(P, T, epochs=per, batch=4096, lr_max=lr, lr_min=1e-5))
         else:
@@ -261,7 +261,7 @@ def bench_surrogate(args):
                   file=sys.stderr, flush=True)
     loss = np.concatenate(losses)
     train_s = time.perf_counter() - t0
-    print(f"[bench] training: {args.sur_epochs} epochs in {train_s:.1f} s, loss {loss[0]:.2e} -> "
+    print(f"[bench] training: {per * segs} epochs in {train_s:.1f} s, loss {loss[0]:.2e} -> "
           f"{loss[-1]:.2e}", file=sys.stderr, flush=True)
     cal = sg.sample_queries(16384, sg.SEED_CALIB)
     cal_t, _ = sg.solve_queries(ctx, cal, chunk=8192)
@@ -309,7 +309,7 @@ def bench_surrogate(args):
            "data": "synthetic (seeded splitmix64); training data from the GPU solver",
            "config": {"workload": f"surrogate ({sg.DEFAULT_SPACE} design space): "
                                   f"{2 * n_solves} training rows, 4 x (4 x 512 GELU) "
-                                  f"ensemble, {args.sur_epochs} epochs, {nq} held-out queries",
+                                  f"ensemble, {per * segs} epochs, {nq} held-out queries",
                       "batch": nq},
            "max_abs_dRdT_vs_plain_after_fallback": err1,
            "fallback_fraction": float(flagged1.mean()),
@@ -330,7 +330,8 @@ def bench_surrogate(args):
                          "status_nonzero": int((pst != 0).sum())},
            "speedup_vs_plain": value / plain_rate,
            "data_generation": {"solves": n_solves, "s": gen_s, "solves_per_s": n_solves / gen_s},
-           "training": {"epochs": per * segs, "segments": segs, "s": train_s,
+           "training": {"epochs": per * segs, "segments": segs, "lr_decay": args.sur_lr_decay,
+                        "s": train_s,
                         "gpu_ms": train_gpu_ms,
                         "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
                         "tflops": 6.0 * sur.net.P * sur.net.members * 2 * n_solves *
@@ -358,11 +359,14 @@ def main():
                          "padding shorter stacks with zero-thickness exit-medium layers")
     ap.add_argument("--sur-solves", type=int, default=1 << 19,
                     help="surrogate workload: solves of training data (two rows each)")
-    ap.add_argument("--sur-epochs", type=int, default=150,
+    ap.add_argument("--sur-epochs", type=int, default=800,
                     help="surrogate workload: training epochs")
-    ap.add_argument("--sur-segments", type=int, default=1,
+    ap.add_argument("--sur-segments", type=int, default=8,
                     help="surrogate workload: training in this many cosine schedules (warm "
                          "restarts), a progress line after each")
+    ap.add_argument("--sur-lr-decay", type=float, default=0.6,
+                    help="surrogate workload: factor on the peak learning rate from one "
+                         "training segment to the next")
     ap.add_argument("--workload", choices=["pillar2d", "large2d", "uneven", "surrogate"],
                     default="pillar2d",
                     help="pillar2d: the metric's 15x15-order batch; large2d: 31x31 orders, "

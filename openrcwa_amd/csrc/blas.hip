@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void k_getf2_panel(int n, int k0, int nb, cd* 
 constexpr int GF_T = 512;
 
 struct GetrfRegState {
-  cd (*s_row)[NB];          // [2][NB]: the pivot row, the displaced row c
+  cd (*s_row)[2][NB];       // [parity][2][NB]: the pivot row, the displaced row c
   double (*s_wv)[GF_T / 64];
   int (*s_wi)[GF_T / 64];
   int n, k0, nb, r, t, lane, wave, b;
@@ -796,15 +796,15 @@ __device__ __forceinline__ void getrf_reg_step(const GetrfRegState& S, cd (&a)[N
       const int put = (r == bi) ? 0 : ((r == c && bi != c) ? 1 : -1);
       if (put >= 0) {
 #pragma unroll
-        for (int jj = 0; jj < NB; ++jj) S.s_row[put][jj] = a[jj];
+        for (int jj = 0; jj < NB; ++jj) S.s_row[par][put][jj] = a[jj];
       }
       __syncthreads();
       const int take = (bi == c) ? -1 : ((r == c) ? 0 : ((r == bi) ? 1 : -1));
       if (take >= 0) {
 #pragma unroll
-        for (int jj = 0; jj < NB; ++jj) a[jj] = S.s_row[take][jj];
+        for (int jj = 0; jj < NB; ++jj) a[jj] = S.s_row[par][take][jj];
       }
-      const cd pv = S.s_row[0][J];
+      const cd pv = S.s_row[par][0][J];
       const bool zero = (pv.re == 0.0 && pv.im == 0.0);
       if (S.t == 0) {
         S.ipiv[(long long)S.b * S.n + c] = bi;
@@ -813,10 +813,11 @@ __device__ __forceinline__ void getrf_reg_step(const GetrfRegState& S, cd (&a)[N
       if (S.live && r > c) {
         a[J] = zero ? mk(0.0, 0.0) : a[J] * cinv(pv);
 #pragma unroll
-        for (int jj = J + 1; jj < NB; ++jj) a[jj] -= a[J] * S.s_row[0][jj];
+        for (int jj = J + 1; jj < NB; ++jj) a[jj] -= a[J] * S.s_row[par][0][jj];
       }
-      // s_row is rewritten by the next column's exchange
-      __syncthreads();
+      // (s_row and the wave maxima alternate by column parity: the next
+      // column writes the other half, and the one after writes this half
+      // only after every thread has passed the next column's two barriers)
       getrf_reg_step<J + 1>(S, a);
     }
   }
@@ -824,7 +825,7 @@ __device__ __forceinline__ void getrf_reg_step(const GetrfRegState& S, cd (&a)[N
 
 __global__ __launch_bounds__(GF_T) void k_getf2_reg(int n, int k0, int nb, cd* const* Ap,
                                                     int* ipiv, int* info, const int* info_map) {
-  __shared__ cd s_row[2][NB];
+  __shared__ cd s_row[2][2][NB];
   __shared__ double s_wv[2][GF_T / 64];
   __shared__ int s_wi[2][GF_T / 64];
   GetrfRegState S;

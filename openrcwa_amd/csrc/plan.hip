@@ -1363,31 +1363,29 @@ static int plan_run(orcwa_plan* p, hipStream_t s, bool graph) {
         const int rc = interface_fixup(p, ct, si, recovered, s);
         if (rc != ORCWA_OK) return rc;
       }
-      // A = E + H (in E), B = E - H (in H)
-      launch_sum_diff(s, n, E, H, p->dev<const cd* const*>(st.keff_a), ni);
+      // A = E + H (in E), B = E - H (in H), and A^T into M, B^T into the next
+      // R~^T slot (the two products below add to them)
+      cd* const* Rcur = p->dev<cd* const*>(ct.Rt[si & 1]);
+      cd* const* Rnext = p->dev<cd* const*>(ct.Rt[(si + 1) & 1]);
+      launch_sum_diff_t(s, n, E, H, p->dev<const cd* const*>(st.keff_a), M,
+                        last ? nullptr : Rnext, ni);
       if (p->keep) {
         HIPCHK(hipMemcpyAsync(p->kA[si], p->kE_src, nn * sizeof(cd), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(p->kB[si], p->kH_src, nn * sizeof(cd), hipMemcpyDeviceToDevice, s));
       }
       // R~ = phi R phi, kept transposed (the scaling is symmetric in i, j)
-      cd* const* Rcur = p->dev<cd* const*>(ct.Rt[si & 1]);
-      cd* const* Rnext = p->dev<cd* const*>(ct.Rt[(si + 1) & 1]);
       if (si > 0)
         launch_phase_scale(s, n, p->dev<const cd* const*>(st.keff_b),
                            p->dev<const double*>(st.k0d), Rcur, nullptr, p->d_info + ct.i0, ni);
       // M^T = (F + G)^T = A^T + (B R~)^T; the GEMM reads R~ through R~^T and
       // stores its product transposed
-      launch_transpose(s, n, C(E), M, ni);
       if (si > 0)
         zgemm_batched_prec(p->gemm_prec, s, n, n, n, one, C(H), 0, 1, n, 0, C(Rcur), 0, n, 1, 0,
                            one, M, 0, n, 1, ni);
-      if (!last) {
-        // (F - G)^T = B^T + (A R~)^T
-        launch_transpose(s, n, C(H), Rnext, ni);
-        if (si > 0)
-          zgemm_batched_prec(p->gemm_prec, s, n, n, n, one, C(E), 0, 1, n, 0, C(Rcur), 0, n, 1, 0,
-                             one, Rnext, 0, n, 1, ni);
-      }
+      // (F - G)^T = B^T + (A R~)^T
+      if (!last && si > 0)
+        zgemm_batched_prec(p->gemm_prec, s, n, n, n, one, C(E), 0, 1, n, 0, C(Rcur), 0, n, 1, 0,
+                           one, Rnext, 0, n, 1, ni);
       zgetrf_batched(s, n, M, piv, p->d_info + ct.i0, nullptr, ni);
       // R_new^T = (F + G)^-T (F - G)^T
       if (!last) zgetrs_batched(s, n, n, C(M), piv, Rnext, ni, p->d_perm);

@@ -125,9 +125,10 @@ void launch_phase_scale(hipStream_t s, int n, const cd* const* keff, const doubl
                         cd* const* R, cd* const* T, int* info, int batch);
 void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd* rt,
                double* ref, double* trn, cd* amp, int* info, int max_pol, int batch);
-void launch_transpose(hipStream_t s, int n, const cd* const* src, cd* const* dst, int batch);
-void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, const cd* const* keff_a,
-                     int batch);
+// E <- A = E + D H, H <- B = E - D H (D = diag(1/keff_a) where keff_a is given),
+// At <- A^T and, when Bt is not null, Bt <- B^T
+void launch_sum_diff_t(hipStream_t s, int n, cd* const* E, cd* const* H,
+                       const cd* const* keff_a, cd* const* At, cd* const* Bt, int batch);
 void launch_pv_in(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* Vb,
                   cd* const* out, int batch);
 void launch_pv_out(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* U,

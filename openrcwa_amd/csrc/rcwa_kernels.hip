@@ -7,7 +7,7 @@
 //   k_build_pq   : P*Q of rcwa/Layer.cpp:150-220 + rcwa/WaveEquationSolver.cpp:30
 //   k_build_v    : V = Q W diag(1/keff), rcwa/WaveEquationSolver.cpp:33-36
 //   k_uniform    : closed-form modes of a homogeneous layer
-//   k_phase_scale, k_bdinv_apply, k_transpose, k_sum_diff, k_cascade_tail,
+//   k_phase_scale, k_bdinv_apply, k_sum_diff_t, k_cascade_tail,
 //   k_cascade_chain : pieces of the RTCM cascade and interfaces
 //       rcwa/RCWASolver.cpp:378-614
 //   k_rt         : plane-wave response and per-order efficiencies
@@ -450,37 +450,43 @@ __global__ void k_phase_scale(int n, const cd* const* keff, const double* k0d, c
 // step, and T enters only through two columns, carried right to left:
 //   Tdd e_c = 2 tau_0 (phi_1 2 tau_1 (... (phi_last 2 tau_last e_c))).
 // ---------------------------------------------------------------------------
-__global__ void k_transpose(int n, const cd* const* src, cd* const* dst) {
-  __shared__ cd t[32][33];
-  const cd* S = src[blockIdx.z];
-  cd* D = dst[blockIdx.z];
+// A = E + D H, B = E - D H with D = diag(1/keff_a) where the instance's
+// upper layer is patterned (H holds W_a^-1 P_a V_b there) and D = I otherwise;
+// A and B are written in place and also transposed, A^T into At and B^T into
+// Bt (when Bt is given): the cascade's next operands, in one pass over E and H
+// (32 x 32 tiles through LDS).
+__global__ void k_sum_diff_t(int n, cd* const* Ep, cd* const* Hp, const cd* const* keff_a,
+                             cd* const* Atp, cd* const* Btp) {
+  __shared__ cd ta[32][33], tb[32][33];
+  const int z = blockIdx.z;
+  cd* E = Ep[z];
+  cd* H = Hp[z];
+  const cd* kf = keff_a[z];
+  cd* At = Atp[z];
+  cd* Bt = Btp ? Btp[z] : nullptr;
   const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 256 threads: 32 x 8
   for (int k = ty; k < 32; k += 8) {
     const int r = r0 + tx, c = c0 + k;
-    if (r < n && c < n) t[k][tx] = S[r + (long long)c * n];
+    if (r < n && c < n) {
+      const long long e = r + (long long)c * n;
+      const cd a = E[e];
+      cd h = H[e];
+      if (kf) h = cdiv(h, kf[r]);
+      const cd sa = a + h, sb = a - h;
+      E[e] = sa;
+      H[e] = sb;
+      ta[k][tx] = sa;
+      tb[k][tx] = sb;
+    }
   }
   __syncthreads();
   for (int k = ty; k < 32; k += 8) {
-    const int r = c0 + tx, c = r0 + k;   // D(r, c) = S(c, r)
-    if (r < n && c < n) D[r + (long long)c * n] = t[tx][k];
-  }
-}
-
-// A = E + D H, B = E - D H with D = diag(1/keff_a) where the instance's
-// upper layer is patterned (H holds W_a^-1 P_a V_b there) and D = I otherwise.
-__global__ void k_sum_diff(int n, cd* const* Ep, cd* const* Hp, const cd* const* keff_a) {
-  cd* E = Ep[blockIdx.y];
-  cd* H = Hp[blockIdx.y];
-  const cd* kf = keff_a[blockIdx.y];
-  const long long count = (long long)n * n;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
-       e += (long long)gridDim.x * blockDim.x) {
-    const cd a = E[e];
-    cd c = H[e];
-    if (kf) c = cdiv(c, kf[e % n]);
-    E[e] = a + c;
-    H[e] = a - c;
+    const int r = c0 + tx, c = r0 + k;   // At(r, c) = A(c, r)
+    if (r < n && c < n) {
+      At[r + (long long)c * n] = ta[tx][k];
+      if (Bt) Bt[r + (long long)c * n] = tb[tx][k];
+    }
   }
 }
 
@@ -934,18 +940,11 @@ void launch_rt(hipStream_t s, const ConvGeom& g, const RtParam* params, const cd
                      max_pol);
 }
 
-void launch_transpose(hipStream_t s, int n, const cd* const* src, cd* const* dst, int batch) {
+void launch_sum_diff_t(hipStream_t s, int n, cd* const* E, cd* const* H,
+                       const cd* const* keff_a, cd* const* At, cd* const* Bt, int batch) {
   if (batch <= 0) return;
   const unsigned t = (unsigned)((n + 31) / 32);
-  hipLaunchKernelGGL(k_transpose, dim3(t, t, batch), dim3(256), 0, s, n, src, dst);
-}
-
-void launch_sum_diff(hipStream_t s, int n, cd* const* E, cd* const* H, const cd* const* keff_a,
-                     int batch) {
-  if (batch <= 0) return;
-  long long b = blocks_for((long long)n * n);
-  hipLaunchKernelGGL(k_sum_diff, dim3((unsigned)(b > 2048 ? 2048 : b), batch), dim3(256), 0, s, n,
-                     E, H, keff_a);
+  hipLaunchKernelGGL(k_sum_diff_t, dim3(t, t, batch), dim3(256), 0, s, n, E, H, keff_a, At, Bt);
 }
 
 void launch_pv_in(hipStream_t s, const ConvGeom& g, const ProbParam* params, const cd* const* Vb,

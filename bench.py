@@ -241,7 +241,7 @@ def bench_surrogate(args):
     gen_s = time.perf_counter() - t0
     print(f"[bench] surrogate data: {n_solves} solves ({2 * n_solves} rows) in {gen_s:.1f} s",
           file=sys.stderr, flush=True)
-    sur = sg.Surrogate(ctx, members=4, width=512, depth=4, seed=21)
+    sur = sg.Surrogate(ctx, members=args.sur_members, width=512, depth=4, seed=21)
     t0 = time.perf_counter()
     # --sur-segments k: k cosine schedules of epochs / k each, the peak
     # learning rate scaled by --sur-lr-decay from one to the next (warm restarts)
@@ -308,7 +308,7 @@ def bench_surrogate(args):
            "vs_baseline": None, "dtype": "f32 network (FP32 MFMA), f64 fallback solves",
            "data": "synthetic (seeded splitmix64); training data from the GPU solver",
            "config": {"workload": f"surrogate ({sg.DEFAULT_SPACE} design space): "
-                                  f"{2 * n_solves} training rows, 4 x (4 x 512 GELU) "
+                                  f"{2 * n_solves} training rows, {args.sur_members} x (4 x 512 GELU) "
                                   f"ensemble, {per * segs} epochs, {nq} held-out queries",
                       "batch": nq},
            "max_abs_dRdT_vs_plain_after_fallback": err1,
@@ -361,6 +361,8 @@ def main():
                     help="surrogate workload: solves of training data (two rows each)")
     ap.add_argument("--sur-epochs", type=int, default=800,
                     help="surrogate workload: training epochs")
+    ap.add_argument("--sur-members", type=int, default=4,
+                    help="surrogate workload: networks in the ensemble")
     ap.add_argument("--sur-segments", type=int, default=8,
                     help="surrogate workload: training in this many cosine schedules (warm "
                          "restarts), a progress line after each")

@@ -20,6 +20,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "layer-stack solves/sec at 2D 15x15 orders; max |dR,dT| vs FP64 oracle"
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and MFMA)
+FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 MFMA
 # HBM bytes per 450 x 450 matrix of the eigensolver stage: 1.969 x FETCH_SIZE +
 # WRITE_SIZE over its dispatches (two rocprofv3 --pmc runs, the factor from the
 # plain copy kernel of the same run; profiles/pillar2d_eigensolver_pmc_traffic.md)
@@ -300,7 +301,15 @@ def bench_surrogate(args):
     bf16_ms = sur.net.last_ms()
     sur.net.set_precision(False)
     flop = 2.0 * sur.net.P * sur.net.members * nq
+    net_tflops = flop / (net_gpu_ms * 1e-3) / 1e12
     value = nq / tm1["total_s"]
+    # roofline of the network's forward pass (the FP32 MFMA products of every
+    # member over all queries, timed with HIP events on the engine's stream);
+    # the fallback solves are the plain path, measured by the pillar2d line
+    roof = {"bound": "mfma", "achieved": net_tflops, "peak": FP32_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": net_tflops / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
+            "kernel": f"network forward pass, {nq} queries x {sur.net.members} members, "
+                      f"{flop:.3e} flop per launch sequence"}
     out = {"metric": "surrogate queries/sec with plain-solver fallback at 1e-3 (2-layer 2D "
                      "grating, 11x11 orders); max |dR,dT| vs plain GPU solver",
            "value": value, "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -311,6 +320,7 @@ def bench_surrogate(args):
                                   f"{2 * n_solves} training rows, {args.sur_members} x (4 x 512 GELU) "
                                   f"ensemble, {per * segs} epochs, {nq} held-out queries",
                       "batch": nq},
+           "roofline": roof,
            "max_abs_dRdT_vs_plain_after_fallback": err1,
            "fallback_fraction": float(flagged1.mean()),
            "calibration": {"spread_threshold": tau1, "fallback_fraction": cal1,
@@ -318,7 +328,7 @@ def bench_surrogate(args):
            "by_tolerance": per_tol,
            "network_only": {"queries_per_s": nq / tm1["network_s"],
                             "gpu_ms": net_gpu_ms,
-                            "tflops": flop / (net_gpu_ms * 1e-3) / 1e12,
+                            "tflops": net_tflops,
                             "max_abs_err": float(net_err.max()),
                             "median_abs_err": float(np.median(net_err)),
                             "p90_abs_err": float(np.percentile(net_err, 90)),

@@ -283,19 +283,37 @@ __global__ void k_mse(const float* Y, const float* T, int bs, int nout, int nout
   }
 }
 
-// db[m][n] = sum over rows of G[m][r][n]
-__global__ void k_colsum(const float* G, int rows, int N, long long sgbat, float* db,
-                         long long sdbat) {
-  const int m = blockIdx.y, c = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
-  float s = 0.f;
-  if (c < N)
-    for (int r = r0; r < rows; r += 4) s += G[m * sgbat + (long long)r * N + c];
-  __shared__ float red[4][64];
-  red[r0][threadIdx.x & 63] = s;
+// db[m][n] = sum over rows of G[m][r][n].  A block takes 16 columns of one
+// member (a wave reads four 64-byte row segments per load), so the launch
+// has N/16 x members blocks; its 16 row groups keep four independent
+// partial sums each (loads in flight instead of one dependent add chain),
+// combined in a fixed order: the result does not depend on timing.
+constexpr int CS_COLS = 16;
+__global__ __launch_bounds__(256) void k_colsum(const float* G, int rows, int N,
+                                                long long sgbat, float* db, long long sdbat) {
+  const int m = blockIdx.y, cl = threadIdx.x % CS_COLS, rg = threadIdx.x / CS_COLS;
+  const int c = blockIdx.x * CS_COLS + cl;
+  constexpr int RG = 256 / CS_COLS;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  if (c < N) {
+    const float* g = G + m * sgbat + c;
+    int r = rg;
+    for (; r + 3 * RG < rows; r += 4 * RG) {
+      s0 += g[(long long)r * N];
+      s1 += g[(long long)(r + RG) * N];
+      s2 += g[(long long)(r + 2 * RG) * N];
+      s3 += g[(long long)(r + 3 * RG) * N];
+    }
+    for (; r < rows; r += RG) s0 += g[(long long)r * N];
+  }
+  __shared__ float red[RG][CS_COLS + 1];
+  red[rg][cl] = (s0 + s1) + (s2 + s3);
   __syncthreads();
-  if (r0 == 0 && c < N)
-    db[m * sdbat + c] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
-                        red[3][threadIdx.x];
+  if (rg == 0 && c < N) {
+    float t = 0.f;
+    for (int k = 0; k < RG; ++k) t += red[k][cl];
+    db[m * sdbat + c] = t;
+  }
 }
 
 struct AdamArgs {
@@ -499,7 +517,7 @@ void backward(orcwa_mlp* n, hipStream_t s, const float* x, int rows) {
     g.ldc = N;
     g.scbat = n->P;
     launch_gemm<false, true, EPI_STORE>(s, g, M);
-    hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64, M), dim3(256), 0, s, G, rows, N,
+    hipLaunchKernelGGL(k_colsum, dim3((N + CS_COLS - 1) / CS_COLS, M), dim3(256), 0, s, G, rows, N,
                        (long long)rows * N, n->d_grad + n->offB[l], n->P);
     if (l > 0) {
       // G_{l-1} = (G W_l^T) * GELU'(Z_{l-1}): C [rows][K]

@@ -24,7 +24,7 @@ FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 MFMA
 # HBM bytes per 450 x 450 matrix of the eigensolver stage: 1.969 x FETCH_SIZE +
 # WRITE_SIZE over its dispatches (two rocprofv3 --pmc runs, the factor from the
 # plain copy kernel of the same run; profiles/pillar2d_eigensolver_pmc_traffic.md)
-EIG_TRAFFIC_PER_MATRIX_N450 = 1.734e9
+EIG_TRAFFIC_PER_MATRIX_N450 = 1.720e9
 
 
 def eig_flops(n):

@@ -402,8 +402,9 @@ def main():
                     help="skip the host-to-host (plan + copies + execute) timing")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the batch shards "
-              "across GPUs by instance, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the batch would "
+              "shard across GPUs by instance, one plan per GPU, with no multi-GPU run here",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -31,6 +31,7 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64, BN = 64, BK = 16;    // FP32 tile
 constexpr int BKH = 32;                     // BF16 tile depth
@@ -175,8 +176,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 // k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8; D as the FP32 form.
 template <int EPI>
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
-  __shared__ __bf16 Ah[BM][LDH];
-  __shared__ __bf16 Bh[BN][LDH];
+  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM][LDH];
+  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN][LDH];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int z = blockIdx.z;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
@@ -187,22 +188,44 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
     for (int b = 0; b < 2; ++b) acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < g.K; k0 += BKH) {
-    // A tile 64 x 32: two float4 per thread along k
+  // The next K step's operands are loaded into registers while the current
+  // one is multiplied.  A: two float4 per thread along k (rows of K floats);
+  // B: four rows k..k+3 of one column n per thread and q, so each load of a
+  // wave reads 64 consecutive floats of a row and each thread stores its four
+  // values as one 8-byte run of the k-fastest LDS row.
+  f4 ra[2], rb[2];
+  auto load = [&](int k0) {
     for (int q = 0; q < 2; ++q) {
       const int idx = t + q * 256, mm = idx >> 3, kq = (idx & 7) * 4;
-      f4 v = {0.f, 0.f, 0.f, 0.f};
-      if (m0 + mm < g.M && k0 + kq < g.K) v = *(const f4*)(A + (m0 + mm) * g.sam + k0 + kq);
-      for (int e = 0; e < 4; ++e) Ah[mm][kq + e] = (__bf16)v[e];
+      ra[q] = (f4){0.f, 0.f, 0.f, 0.f};
+      if (m0 + mm < g.M && k0 + kq < g.K) ra[q] = *(const f4*)(A + (m0 + mm) * g.sam + k0 + kq);
     }
-    // B tile 32 x 64: two float4 per thread along n, stored n-major
     for (int q = 0; q < 2; ++q) {
-      const int idx = t + q * 256, kk = idx >> 4, nq = (idx & 15) * 4;
-      f4 v = {0.f, 0.f, 0.f, 0.f};
-      if (k0 + kk < g.K && n0 + nq < g.N) v = *(const f4*)(B + (k0 + kk) * g.sbk + n0 + nq);
-      for (int e = 0; e < 4; ++e) Bh[nq + e][kk] = (__bf16)v[e];
+      const int idx = t + q * 256, nn = idx & 63, kq = (idx >> 6) * 4;
+      for (int e = 0; e < 4; ++e)
+        rb[q][e] = (k0 + kq + e < g.K && n0 + nn < g.N)
+                       ? B[(long long)(k0 + kq + e) * g.sbk + n0 + nn] : 0.f;
     }
+  };
+  auto stage = [&]() {
+    for (int q = 0; q < 2; ++q) {
+      const int idx = t + q * 256, mm = idx >> 3, kq = (idx & 7) * 4;
+      bf4 h;
+      for (int e = 0; e < 4; ++e) h[e] = (__bf16)ra[q][e];
+      *(bf4*)&Ah[mm][kq] = h;
+    }
+    for (int q = 0; q < 2; ++q) {
+      const int idx = t + q * 256, nn = idx & 63, kq = (idx >> 6) * 4;
+      bf4 h;
+      for (int e = 0; e < 4; ++e) h[e] = (__bf16)rb[q][e];
+      *(bf4*)&Bh[nn][kq] = h;
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < g.K; k0 += BKH) {
+    stage();
     __syncthreads();
+    if (k0 + BKH < g.K) load(k0 + BKH);
     bf8 av[2], bv[2];
     for (int a = 0; a < 2; ++a) av[a] = *(const bf8*)&Ah[wm + a * 16 + li][lk * 8];
     for (int b = 0; b < 2; ++b) bv[b] = *(const bf8*)&Bh[wn + b * 16 + li][lk * 8];

@@ -135,3 +135,58 @@ def test_features_layout():
     f = sg.features(q)
     assert f.shape == (64, sg.N_FEAT) and f.dtype == np.float32
     assert np.array_equal(f[:, :sg.N_IN], sg.scale_inputs(q))
+
+
+class _StubNet:
+    """Numpy stand-in for the GPU ensemble in calibration: fixed member outputs
+    (members, rows, n_out) for the rows it is asked about, and the device's
+    error estimate max_c kappa_c (max - min over members of output c)."""
+
+    def __init__(self, y):
+        self.y = y
+        self.kappa = None
+
+    def forward(self, x):
+        assert x.shape[0] == self.y.shape[1]
+        return self.y
+
+    def set_error_scale(self, kappa):
+        self.kappa = np.asarray(kappa, dtype=np.float64)
+
+    def predict(self, x):
+        assert x.shape[0] == self.y.shape[1]
+        spread = self.y.max(0) - self.y.min(0)
+        return self.y.mean(0), (self.kappa[None, :] * spread).max(1)
+
+
+def test_calibration_flags_every_calibration_miss():
+    rng = np.random.default_rng(5)
+    rows, tol = 400, 1e-3
+    params = sg.sample_queries(rows, 99)
+    t = rng.uniform(0.0, 0.05, size=(rows, sg.N_OUT))
+    s = 10.0 ** rng.uniform(-5, -2, size=rows)            # per-query member spread
+    off = np.full(rows, 0.3 * tol)                         # error of the ensemble mean
+    bad = rng.permutation(rows)[:30]
+    off[bad] = 2.0 * tol
+    unphys = 7                                             # a negative answer, tiny spread
+    bad = bad[bad != unphys]
+    s[unphys] = 1e-7
+    d = np.array([-0.5, -0.1, 0.1, 0.5])                  # member offsets: mean 0, range 1
+    y = t[None] + d[:, None, None] * s[None, :, None]
+    y[:, :, 0] += off[None, :]
+    y[:, unphys, 0] = -3.0 * tol + d * s[unphys]
+    sur = sg.Surrogate.__new__(sg.Surrogate)
+    sur.space, sur.tol, sur.kappa, sur.tau = sg.DEFAULT_SPACE, tol, None, None
+    sur.net = _StubNet(y)
+    tau, frac = sur.calibrate(params, t, margin=1.5, tol=tol)
+    # tau: the smallest spread among the physical misses, over the margin; the
+    # unphysical answer does not set it (its spread is far smaller)
+    assert tau == pytest.approx(s[bad].min() / 1.5, rel=1e-12)
+    mean, est = sur.net.predict(np.zeros((rows, 1)))
+    flagged = sur._flags(mean, est)
+    assert flagged[bad].all()
+    assert flagged[unphys]
+    good = np.setdiff1d(np.arange(rows), np.append(bad, unphys))
+    assert not flagged[good[s[good] < 0.99 * tau]].any()
+    assert flagged[good[s[good] > 1.01 * tau]].all()
+    assert frac == pytest.approx(flagged.mean())
